@@ -1,0 +1,196 @@
+/*
+ * qfe.h -- C ABI of the MI355X state-vector engine behind qforte_amd.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes and a
+ * thread-local message (qfe_last_error).  No C++ exception ever crosses this
+ * boundary and no torch type appears in a signature.  Amplitudes are complex128
+ * stored as interleaved (re, im) doubles, index bit q = qubit q, exactly the
+ * layout of the reference Computer's std::vector<std::complex<double>>
+ * (src/qforte/computer.h:176, qubit_basis.h:27-36), so upload/download are
+ * plain copies.
+ *
+ * Each entry point names the reference interface it replaces.  Entry points
+ * with no counterpart in the reference say so and name the callers they
+ * serve.
+ */
+#ifndef QFE_H
+#define QFE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define QFE_OK 0
+#define QFE_ERR_ARG 1      /* invalid argument (bad handle, size mismatch, qubit out of range) */
+#define QFE_ERR_DEVICE 2   /* HIP runtime error, or no GPU present                           */
+#define QFE_ERR_NOMEM 3    /* device allocation failed                                        */
+#define QFE_ERR_STATE 4    /* operation not valid in the current state                        */
+
+typedef struct qfe_state_s* qfe_state;     /* one 2^n complex128 state resident in HBM      */
+typedef struct qfe_program_s* qfe_program; /* compiled Pauli programs resident in HBM       */
+typedef struct qfe_batch_s* qfe_batch;     /* serving engine (continuous batching)          */
+
+typedef struct {
+    double re, im;
+} qfe_c64;
+
+/* A gate in the reference's own form: target, control and the 4x4 matrix in
+ * the basis 2*control_bit + target_bit, row-major (gate.h:52-154; a 1-qubit
+ * gate has target == control and uses the top-left 2x2 block). */
+typedef struct {
+    int32_t target;
+    int32_t control;
+    qfe_c64 m[16];
+} qfe_gate;
+
+/* A Pauli operator in mask form: P|j> = phase * (-1)^popcount(j & z) |j ^ x>.
+ * coeff multiplies P.  Compiled on the host from a canonical Pauli Circuit
+ * (circuit.cc:107-169); phase is one of +1, -1, +i, -i. */
+typedef struct {
+    uint64_t x;
+    uint64_t z;
+    qfe_c64 phase;
+    qfe_c64 coeff;
+} qfe_pauli;
+
+/* One element of an operation stream for qfe_apply_ops. */
+#define QFE_OP_GATE 0 /* a gate (uses .gate)                                                  */
+#define QFE_OP_PEXP 1 /* exp(i * angle * P) for a Hermitian P (uses .pauli, .angle)            */
+typedef struct {
+    int32_t kind;
+    int32_t pad;
+    double angle;
+    qfe_gate gate;
+    qfe_pauli pauli;
+} qfe_op;
+
+/* ---- library ------------------------------------------------------------ */
+/* No reference counterpart (the reference has no device): error reporting. */
+const char* qfe_last_error(void);
+/* Select the GPU and create the library stream.  Called implicitly by the
+ * first state creation; no reference counterpart. */
+int qfe_init(int device);
+/* Number of visible GPUs (0 on a host without one); no reference counterpart. */
+int qfe_device_count(int* count);
+/* The hipStream_t every kernel of the library runs on (for event timing by
+ * callers); no reference counterpart. */
+int qfe_get_stream(void** stream);
+/* Block until all queued work finished. */
+int qfe_synchronize(void);
+/* Fusion / launch tuning: max tile qubits (k), low contiguous qubits (L) and
+ * max ops per fused pass.  Zero leaves a value unchanged.  No reference
+ * counterpart (autotuner hook). */
+int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass);
+int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass);
+/* Counters of launched passes since the last reset (fused passes, single-op
+ * passes, expectation passes) and the algorithmic bytes they moved. */
+int qfe_get_counters(uint64_t* fused_passes, uint64_t* single_passes, uint64_t* reduce_passes, double* bytes);
+int qfe_reset_counters(void);
+
+/* ---- state (Computer storage, computer.cc:28-54, computer.h:28,112,132) -- */
+/* Computer::Computer(nqubit): allocate and set |0...0> (computer.cc:28-38). */
+int qfe_state_create(int nqubit, qfe_state* out);
+int qfe_state_destroy(qfe_state s);
+int qfe_state_nqubit(qfe_state s, int* nqubit);
+/* Computer(const Computer&) deep copy (computer.h:28): device-to-device. */
+int qfe_state_copy(qfe_state dst, qfe_state src);
+/* set_coeff_vec / set_coeff_vec_from_numpy (computer.h:132, bindings.cc:176-182). */
+int qfe_state_upload(qfe_state s, const qfe_c64* host, uint64_t namps);
+/* get_coeff_vec (computer.h:112). */
+int qfe_state_download(qfe_state s, qfe_c64* host, uint64_t namps);
+/* reset (computer.cc:51-54). */
+int qfe_state_reset(qfe_state s);
+/* null_state (computer.cc:49). */
+int qfe_state_zero(qfe_state s);
+/* set_state: zero, then scatter (index, value) pairs (computer.cc:42-47). */
+int qfe_state_set_amps(qfe_state s, const uint64_t* idx, const qfe_c64* val, uint64_t count);
+/* coeff(QubitBasis) (computer.cc:40). */
+int qfe_state_get_amp(qfe_state s, uint64_t idx, qfe_c64* out);
+/* apply_constant (computer.cc:130-133). */
+int qfe_state_scale(qfe_state s, qfe_c64 a);
+/* y += a * x; the accumulate step of apply_operator (computer.cc:84-86). */
+int qfe_state_axpy(qfe_state y, qfe_c64 a, qfe_state x);
+/* dot(qc1, qc2) = sum conj(a_i) b_i (computer.cc:975-980), deterministic order. */
+int qfe_state_dot(qfe_state a, qfe_state b, qfe_c64* out);
+/* operator== (computer.cc:968-973): exact equality of every amplitude. */
+int qfe_state_equal(qfe_state a, qfe_state b, int* equal);
+/* Probability scan sum |c_i|^2 (norm squared); no direct reference counterpart,
+ * used by norm checks. */
+int qfe_state_norm2(qfe_state s, double* out);
+
+/* ---- gates and Pauli exponentials --------------------------------------- */
+/* apply_gate / apply_circuit (computer.cc:93-114, 340-751).  Gates are fused
+ * into multi-gate passes where that is exact. */
+int qfe_apply_gates(qfe_state s, const qfe_gate* gates, uint64_t count);
+/* A mixed stream of gates and Pauli exponentials, applied in order.  A
+ * QFE_OP_PEXP replaces the gate sequence exponentiate_pauli_string builds
+ * for exp(i*angle*P) (exponentiate.py:9-87) by one pass. */
+int qfe_apply_ops(qfe_state s, const qfe_op* ops, uint64_t count);
+
+/* ---- Pauli sums (computer.cc:80-91, 753-806) ----------------------------- */
+/* direct_op_exp_val for an operator of Pauli terms: <psi| sum_k coeff_k P_k |psi>. */
+int qfe_expval(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out);
+/* Several operators at once (direct_oppl_exp_val, computer.cc:773-806):
+ * op_offsets[k]..op_offsets[k+1] delimit operator k; out[k] = <psi|O_k|psi>. */
+int qfe_expval_many(qfe_state s, const qfe_pauli* terms, const uint64_t* op_offsets, uint64_t nops,
+                    qfe_c64* out);
+/* apply_operator: out = sum_k coeff_k P_k in (computer.cc:80-91).  out != in. */
+int qfe_apply_pauli_sum(qfe_state in, const qfe_pauli* terms, uint64_t count, qfe_state out);
+/* <bra| sum_k coeff_k P_k |ket>: the np.vdot(sigma, K psi) of
+ * measure_gradient / measure_gradient3 (uccvqeabc.py:159-162, 242-244,
+ * 274-278) without forming K psi. */
+int qfe_cross_expval(qfe_state bra, qfe_state ket, const qfe_pauli* terms, uint64_t count, qfe_c64* out);
+
+/* ---- compiled UCC programs (no single reference counterpart) ------------
+ * A program holds, resident in HBM, the exponential sequence of a
+ * disentangled UCC ansatz (ansatz.py:25-82 + trotterization.py:10-57) and a
+ * Hamiltonian (Molecule.hamiltonian).  Amplitudes are runtime inputs, so one
+ * compile serves every energy and gradient call of an optimisation
+ * (energy_feval, algorithm.py:458-474; measure_gradient, uccvqeabc.py:122-251).
+ *
+ * Generators: generator k has terms gen_offsets[k]..gen_offsets[k+1] (each
+ * term's coeff is the purely imaginary JW coefficient); the ansatz applies
+ * exp(t[param[k]] * scale[k] * sum_j coeff_j P_j) for k = 0, 1, ... with the
+ * terms in order, as trotterize does. */
+int qfe_program_create(int nqubit, uint64_t ref_index, const qfe_pauli* gen_terms, const uint64_t* gen_offsets,
+                       uint64_t ngen, const int32_t* gen_param, const double* gen_scale, uint64_t nparam,
+                       const qfe_pauli* ham_terms, uint64_t nham, qfe_program* out);
+int qfe_program_destroy(qfe_program p);
+/* E(t) = <ref|U(t)^dag H U(t)|ref> (energy_feval, algorithm.py:458-474). */
+int qfe_program_energy(qfe_program p, const double* params, double* energy);
+/* Prepare U(t)|ref> into a state (build_Uvqc + apply_circuit). */
+int qfe_program_prepare(qfe_program p, const double* params, qfe_state out);
+/* Adjoint gradient dE/dt_k (measure_gradient, uccvqeabc.py:122-251): one
+ * forward sweep, sigma = H psi, then a reverse sweep un-applying each
+ * exponential on psi and sigma.  grad has nparam entries; energy may be NULL. */
+int qfe_program_gradient(qfe_program p, const double* params, double* grad, double* energy);
+/* ADAPT pool gradients 2 Re <H psi | A_k psi> for many operators A_k at the
+ * state U(t)|ref> (measure_gradient3, uccvqeabc.py:253-283).  Operators are
+ * delimited by op_offsets; processed in batches of up to `batch` operators
+ * per launch. */
+int qfe_program_pool_gradients(qfe_program p, const double* params, const qfe_pauli* pool_terms,
+                               const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
+
+/* ---- serving: continuous batching of UCC energy requests ----------------
+ * No reference counterpart: the reference evaluates one energy per Python
+ * call.  A batch engine admits requests (a compiled program plus its
+ * amplitudes) while its HBM budget allows, advances every admitted request
+ * together, and hands results back as they complete. */
+int qfe_batch_create(uint64_t hbm_budget_bytes, int max_active, qfe_batch* out);
+int qfe_batch_destroy(qfe_batch b);
+/* Queue a request; returns its id.  The program must outlive the request. */
+int qfe_batch_submit(qfe_batch b, qfe_program p, const double* params, uint64_t* request_id);
+/* Run the engine until at least one request completes or nothing is
+ * pending; completed results are returned (ids, energies, up to cap). */
+int qfe_batch_step(qfe_batch b, uint64_t* done_ids, double* done_energies, uint64_t cap, uint64_t* ndone);
+int qfe_batch_pending(qfe_batch b, uint64_t* queued, uint64_t* active);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QFE_H */

@@ -1,0 +1,447 @@
+// Python module qforte_amd._core: the qforte API surface of the reference's
+// pybind11 module (bindings.cc:23-350) over the native host core, plus the
+// compiled-program and serving entry points that sit beside it.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <algorithm>
+#include <memory>
+
+#include "core.h"
+
+namespace py = pybind11;
+using namespace pybind11::literals;
+using namespace qfa;
+
+namespace {
+
+const char* kOneQubit[] = {"X", "Y", "Z", "H", "R", "Rx", "Ry", "Rz", "V", "S", "T", "I"};
+const char* kTwoQubitNoParam[] = {"SWAP", "cV", "CNOT", "cX", "aCNOT", "acX", "cY", "cZ"};
+const char* kOneQubitNoParam[] = {"X", "Y", "Z", "H", "V", "S", "T", "I"};
+
+template <size_t N>
+bool in(const std::string& s, const char* (&list)[N]) {
+    return std::any_of(std::begin(list), std::end(list), [&](const char* t) { return s == t; });
+}
+
+py::array_t<std::complex<double>> coeff_array(const Computer& c) {
+    py::array_t<std::complex<double>> out(c.get_nbasis());
+    {
+        py::gil_scoped_release rel;
+        c.get_coeff_into(out.mutable_data());
+    }
+    return out;
+}
+
+void set_coeffs_from(Computer& c, py::object obj) {
+    auto arr = py::array_t<std::complex<double>, py::array::c_style | py::array::forcecast>::ensure(obj);
+    if (!arr) throw std::invalid_argument("set_coeff_vec: expected a sequence of complex numbers");
+    py::gil_scoped_release rel;
+    c.set_coeff_vec(arr.data(), (size_t)arr.size());
+}
+
+// ---- compiled UCC program -----------------------------------------------------------
+class UCCProgram {
+  public:
+    UCCProgram(int nqubit, uint64_t ref_index, const std::vector<QubitOperator>& generators,
+               const std::vector<int>& param_index, const std::vector<double>& scale, size_t nparam,
+               const QubitOperator& hamiltonian)
+        : n_(nqubit), nparam_(nparam) {
+        if (generators.size() != param_index.size() || generators.size() != scale.size())
+            throw std::invalid_argument("UCCProgram: generators, param_index and scale must have equal length");
+        std::vector<qfe_pauli> terms, one;
+        std::vector<uint64_t> off{0};
+        for (const auto& g : generators) {
+            if (!compile_pauli_sum(g, one)) throw std::invalid_argument("UCCProgram: generators must be sums of Pauli strings");
+            terms.insert(terms.end(), one.begin(), one.end());
+            off.push_back(terms.size());
+        }
+        std::vector<qfe_pauli> ham;
+        if (!compile_pauli_sum(hamiltonian, ham)) throw std::invalid_argument("UCCProgram: the Hamiltonian must be a sum of Pauli strings");
+        std::vector<int32_t> pi(param_index.begin(), param_index.end());
+        check(qfe_program_create(nqubit, ref_index, terms.data(), off.data(), generators.size(), pi.data(), scale.data(),
+                                 nparam, ham.data(), ham.size(), &p_),
+              "UCCProgram");
+    }
+    ~UCCProgram() {
+        if (p_) qfe_program_destroy(p_);
+    }
+    UCCProgram(const UCCProgram&) = delete;
+    UCCProgram& operator=(const UCCProgram&) = delete;
+
+    double energy(const std::vector<double>& t) {
+        need(t);
+        double e = 0;
+        py::gil_scoped_release rel;
+        check(qfe_program_energy(p_, t.data(), &e), "UCCProgram.energy");
+        return e;
+    }
+    std::pair<double, std::vector<double>> gradient(const std::vector<double>& t) {
+        need(t);
+        std::vector<double> g(nparam_);
+        double e = 0;
+        py::gil_scoped_release rel;
+        check(qfe_program_gradient(p_, t.data(), g.data(), &e), "UCCProgram.gradient");
+        return {e, g};
+    }
+    void prepare(const std::vector<double>& t, Computer& out) {
+        need(t);
+        py::gil_scoped_release rel;
+        check(qfe_program_prepare(p_, t.data(), out.handle()), "UCCProgram.prepare");
+    }
+    std::vector<double> pool_gradients(const std::vector<double>& t, const std::vector<QubitOperator>& ops,
+                                       size_t batch) {
+        need(t);
+        std::vector<qfe_pauli> terms, one;
+        std::vector<uint64_t> off{0};
+        for (const auto& g : ops) {
+            if (!compile_pauli_sum(g, one)) throw std::invalid_argument("pool_gradients: operators must be Pauli sums");
+            terms.insert(terms.end(), one.begin(), one.end());
+            off.push_back(terms.size());
+        }
+        std::vector<double> out(ops.size());
+        py::gil_scoped_release rel;
+        check(qfe_program_pool_gradients(p_, t.data(), terms.data(), off.data(), ops.size(), batch, out.data()),
+              "UCCProgram.pool_gradients");
+        return out;
+    }
+    int nqubit() const { return n_; }
+    size_t nparam() const { return nparam_; }
+    qfe_program handle() const { return p_; }
+
+  private:
+    void need(const std::vector<double>& t) const {
+        if (t.size() != nparam_) throw std::invalid_argument("UCCProgram: expected " + std::to_string(nparam_) + " amplitudes");
+    }
+    int n_;
+    size_t nparam_;
+    qfe_program p_ = nullptr;
+};
+
+class BatchEngine {
+  public:
+    BatchEngine(uint64_t budget, int max_active) { check(qfe_batch_create(budget, max_active, &b_), "BatchEngine"); }
+    ~BatchEngine() {
+        if (b_) qfe_batch_destroy(b_);
+    }
+    uint64_t submit(std::shared_ptr<UCCProgram> p, const std::vector<double>& t) {
+        if (t.size() != p->nparam()) throw std::invalid_argument("BatchEngine.submit: wrong number of amplitudes");
+        uint64_t id = 0;
+        check(qfe_batch_submit(b_, p->handle(), t.data(), &id), "BatchEngine.submit");
+        keep_.push_back(p);
+        return id;
+    }
+    std::vector<std::pair<uint64_t, double>> step(size_t cap) {
+        std::vector<uint64_t> ids(cap);
+        std::vector<double> es(cap);
+        uint64_t nd = 0;
+        {
+            py::gil_scoped_release rel;
+            check(qfe_batch_step(b_, ids.data(), es.data(), cap, &nd), "BatchEngine.step");
+        }
+        std::vector<std::pair<uint64_t, double>> out;
+        for (uint64_t i = 0; i < nd; ++i) out.emplace_back(ids[i], es[i]);
+        uint64_t q = 0, a = 0;
+        qfe_batch_pending(b_, &q, &a);
+        if (q == 0 && a == 0) keep_.clear();
+        return out;
+    }
+    std::pair<uint64_t, uint64_t> pending() {
+        uint64_t q = 0, a = 0;
+        check(qfe_batch_pending(b_, &q, &a), "BatchEngine.pending");
+        return {q, a};
+    }
+
+  private:
+    qfe_batch b_ = nullptr;
+    std::vector<std::shared_ptr<UCCProgram>> keep_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+    m.doc() = "qforte_amd native core: qforte data model over the MI355X state-vector engine";
+
+    py::class_<Circuit, std::shared_ptr<Circuit>>(m, "Circuit")
+        .def(py::init<>())
+        .def(py::init<const Circuit&>())
+        .def("add", &Circuit::add_gate)
+        .def("add", &Circuit::add_circuit)
+        .def("add_gate", &Circuit::add_gate)
+        .def("add_circuit", &Circuit::add_circuit)
+        .def("insert_gate", &Circuit::insert_gate)
+        .def("remove_gate", &Circuit::remove_gate, "pos"_a)
+        .def("swap_gates", &Circuit::swap_gates, "pos1"_a, "pos2"_a)
+        .def("insert_circuit", &Circuit::insert_circuit, "pos"_a, "circ"_a)
+        .def("remove_gates", &Circuit::remove_gates, "pos1"_a, "pos2"_a)
+        .def("replace_gate", &Circuit::replace_gate, "pos"_a, "gate"_a)
+        .def("gates", &Circuit::gates)
+        .def("gate", [](const Circuit& c, size_t pos) { return c.gates().at(pos); })
+        .def("size", &Circuit::size)
+        .def("adjoint", &Circuit::adjoint)
+        .def("canonicalize_pauli_circuit", &Circuit::canonicalize_pauli_circuit)
+        .def("set_parameters", &Circuit::set_parameters)
+        .def("set_parameter", &Circuit::set_parameter)
+        .def("get_parameters", &Circuit::get_parameters)
+        .def("get_num_cnots", &Circuit::get_num_cnots)
+        .def("is_pauli", &Circuit::is_pauli)
+        .def("num_qubits", &Circuit::num_qubits)
+        .def("num_exponential_segments", [](const Circuit& c) { return c.segments().size(); })
+        .def("str", &Circuit::str)
+        .def("__str__", &Circuit::str)
+        .def("__repr__", &Circuit::str)
+        .def("__len__", &Circuit::size)
+        .def("__eq__", [](const Circuit& a, const Circuit& b) { return a == b; });
+
+    py::class_<SQOperator>(m, "SQOperator")
+        .def(py::init<>())
+        .def("add", &SQOperator::add_term)
+        .def("add", &SQOperator::add_op)
+        .def("add_term", &SQOperator::add_term)
+        .def("add_op", &SQOperator::add_op)
+        .def("set_coeffs", &SQOperator::set_coeffs)
+        .def("mult_coeffs", &SQOperator::mult_coeffs)
+        .def("terms", &SQOperator::terms)
+        .def("canonical_order", &SQOperator::canonical_order)
+        .def("simplify", &SQOperator::simplify)
+        .def("jw_transform", &SQOperator::jw_transform, py::arg("qubit_excitation") = false)
+        .def("str", &SQOperator::str)
+        .def("__str__", &SQOperator::str)
+        .def("__repr__", &SQOperator::str);
+
+    py::class_<SQOpPool>(m, "SQOpPool")
+        .def(py::init<>())
+        .def("add", &SQOpPool::add_term)
+        .def("add_term", &SQOpPool::add_term)
+        .def("set_coeffs", &SQOpPool::set_coeffs)
+        .def("terms", &SQOpPool::terms)
+        .def("set_orb_spaces", &SQOpPool::set_orb_spaces, py::arg("ref"), py::arg("orb_irreps_to_int") = std::vector<size_t>{})
+        .def("get_qubit_op_pool", &SQOpPool::get_qubit_op_pool)
+        .def("get_qubit_operator", &SQOpPool::get_qubit_operator, py::arg("order_type"),
+             py::arg("combine_like_terms") = true, py::arg("qubit_excitations") = false)
+        .def("fill_pool", &SQOpPool::fill_pool)
+        .def("str", &SQOpPool::str)
+        .def("__getitem__", [](const SQOpPool& p, size_t i) { return p.terms().at(i); })
+        .def("__iter__", [](const SQOpPool& p) { return py::make_iterator(p.terms()); }, py::keep_alive<0, 1>())
+        .def("__len__", [](const SQOpPool& p) { return p.terms().size(); })
+        .def("__str__", &SQOpPool::str)
+        .def("__repr__", &SQOpPool::str);
+
+    py::class_<QubitOperator>(m, "QubitOperator")
+        .def(py::init<>())
+        .def("add", &QubitOperator::add_term)
+        .def("add", &QubitOperator::add_op)
+        .def("add_term", &QubitOperator::add_term)
+        .def("add_op", &QubitOperator::add_op)
+        .def("set_coeffs", &QubitOperator::set_coeffs)
+        .def("mult_coeffs", &QubitOperator::mult_coeffs)
+        .def("terms", &QubitOperator::terms)
+        .def("order_terms", &QubitOperator::order_terms)
+        .def("canonical_order", &QubitOperator::canonical_order)
+        .def("simplify", &QubitOperator::simplify, py::arg("combine_like_terms") = true)
+        .def("operator_product", &QubitOperator::operator_product, py::arg("rqo"), py::arg("pre_simplify") = true,
+             py::arg("post_simplify") = true)
+        .def("check_op_equivalence", &QubitOperator::check_op_equivalence)
+        .def("num_qubits", &QubitOperator::num_qubits)
+        .def("str", &QubitOperator::str)
+        .def("__iter__", [](const QubitOperator& o) { return py::make_iterator(o.terms()); }, py::keep_alive<0, 1>())
+        .def("__str__", &QubitOperator::str)
+        .def("__repr__", &QubitOperator::str);
+
+    py::class_<QubitOpPool>(m, "QubitOpPool")
+        .def(py::init<>())
+        .def("add", &QubitOpPool::add_term)
+        .def("add_term", &QubitOpPool::add_term)
+        .def("set_coeffs", &QubitOpPool::set_coeffs)
+        .def("set_op_coeffs", &QubitOpPool::set_op_coeffs)
+        .def("set_terms", &QubitOpPool::set_terms)
+        .def("terms", &QubitOpPool::terms)
+        .def("join_op_from_right_lazy", &QubitOpPool::join_op_from_right_lazy)
+        .def("join_op_from_right", &QubitOpPool::join_op_from_right)
+        .def("join_op_from_left", &QubitOpPool::join_op_from_left)
+        .def("join_as_commutator", &QubitOpPool::join_as_commutator)
+        .def("square", &QubitOpPool::square)
+        .def("fill_pool", &QubitOpPool::fill_pool)
+        .def("str", &QubitOpPool::str)
+        .def("__iter__", [](const QubitOpPool& p) { return py::make_iterator(p.terms()); }, py::keep_alive<0, 1>())
+        .def("__str__", &QubitOpPool::str)
+        .def("__repr__", &QubitOpPool::str);
+
+    py::class_<QubitBasis>(m, "QubitBasis")
+        .def(py::init<uint64_t>(), "n"_a = 0)
+        .def("str", &QubitBasis::str)
+        .def("__str__", [](const QubitBasis& b) { return b.str(64); })
+        .def("__repr__", [](const QubitBasis& b) { return b.str(64); })
+        .def("flip_bit", &QubitBasis::flip_bit)
+        .def("set_bit", &QubitBasis::set_bit)
+        .def("index", &QubitBasis::index)
+        .def("get_bit", &QubitBasis::get_bit);
+
+    py::class_<Computer, std::shared_ptr<Computer>>(m, "Computer")
+        .def(py::init<int, double>(), "nqubits"_a, "print_threshold"_a = 1.0e-6)
+        .def(py::init<const Computer&>())
+        .def("apply_circuit_safe", &Computer::apply_circuit_safe)
+        .def("apply_operator", &Computer::apply_operator, py::call_guard<py::gil_scoped_release>())
+        .def("apply_circuit", &Computer::apply_circuit, py::call_guard<py::gil_scoped_release>())
+        .def("apply_gate_safe", &Computer::apply_gate_safe)
+        .def("apply_gate", &Computer::apply_gate, py::call_guard<py::gil_scoped_release>())
+        .def("apply_constant", &Computer::apply_constant, py::call_guard<py::gil_scoped_release>())
+        .def("measure_circuit", &Computer::measure_circuit)
+        .def("measure_z_readouts_fast", &Computer::measure_z_readouts_fast)
+        .def("measure_readouts", &Computer::measure_readouts)
+        .def("perfect_measure_circuit", &Computer::perfect_measure_circuit)
+        .def("direct_oppl_exp_val", &Computer::direct_oppl_exp_val, py::call_guard<py::gil_scoped_release>())
+        .def("direct_idxd_oppl_exp_val", &Computer::direct_idxd_oppl_exp_val, py::call_guard<py::gil_scoped_release>())
+        .def("direct_oppl_exp_val_w_mults", &Computer::direct_oppl_exp_val_w_mults)
+        .def("direct_op_exp_val", &Computer::direct_op_exp_val, py::call_guard<py::gil_scoped_release>())
+        .def("direct_circ_exp_val", &Computer::direct_circ_exp_val)
+        .def("direct_pauli_circ_exp_val", &Computer::direct_pauli_circ_exp_val)
+        .def("coeff", &Computer::coeff)
+        .def("get_coeff_vec", &coeff_array)
+        .def("get_nqubit", &Computer::get_nqubit)
+        .def("set_coeff_vec", &set_coeffs_from)
+        .def("set_coeff_vec_from_numpy", &set_coeffs_from)
+        .def("set_state", &Computer::set_state)
+        .def("null_state", &Computer::null_state)
+        .def("reset", &Computer::reset, "Reset the quantum computer to the state |0>")
+        .def("get_timings", [](const Computer&) { return std::vector<std::pair<std::string, double>>{}; })
+        .def("clear_timings", [](Computer&) {})
+        .def("str", &Computer::str)
+        .def("__str__", &Computer::str)
+        .def("__repr__", &Computer::str)
+        .def("__eq__", [](const Computer& a, const Computer& b) { return a == b; });
+
+    py::class_<Gate>(m, "Gate")
+        .def("gate_id", &Gate::gate_id)
+        .def("target", &Gate::target)
+        .def("control", &Gate::control)
+        .def("adjoint", &Gate::adjoint)
+        .def("nqubits", &Gate::nqubits)
+        .def("str", &Gate::str)
+        .def("has_parameter", &Gate::has_parameter)
+        .def("parameter", &Gate::parameter)
+        .def("matrix", [](const Gate& g) {
+            py::array_t<std::complex<double>> a({4, 4});
+            auto r = a.mutable_unchecked<2>();
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) r(i, j) = g.matrix()[i][j];
+            return a;
+        })
+        .def("__str__", &Gate::str)
+        .def("__repr__", &Gate::repr)
+        .def("__eq__", [](const Gate& a, const Gate& b) { return a == b; })
+        .def("__neq__", [](const Gate& a, const Gate& b) { return a != b; })
+        .def("update_parameter", [](const Gate& g, double p) {
+            if (!g.has_parameter()) throw std::invalid_argument("Gate does not have a parameter.");
+            return make_gate(g.gate_id(), g.target(), g.control(), p);
+        });
+
+    // gate(...) overloads in the reference's registration order (bindings.cc:238-290)
+    m.def(
+        "gate",
+        [](std::string type, size_t target, std::complex<double> parameter) {
+            if (parameter.imag() != 0.0) throw std::invalid_argument("Gate parameter must be real.");
+            if (in(type, kOneQubit)) return make_gate(type, target, target, parameter.real());
+            throw std::invalid_argument("make_gate()\ttarget = " + std::to_string(target) + ", parameter = " +
+                                        std::to_string(parameter.real()) + " is not a valid quantum gate input for type = " + type);
+        },
+        "type"_a, "target"_a, "parameter"_a = 0.0);
+    m.def(
+        "gate",
+        [](std::string type, size_t target, size_t control) {
+            if (in(type, kTwoQubitNoParam)) return make_gate(type, target, control, 0.0);
+            if (target == control && in(type, kOneQubitNoParam)) return make_gate(type, target, control, 0.0);
+            throw std::invalid_argument("make_gate()\ttarget = " + std::to_string(target) + ", control = " +
+                                        std::to_string(control) + ", is not a valid quantum gate input for type = " + type);
+        },
+        "type"_a, "target"_a, "control"_a);
+    m.def(
+        "gate",
+        [](std::string type, size_t target, size_t control, std::complex<double> parameter) {
+            if (parameter.imag() != 0.0) throw std::invalid_argument("Gate parameter must be real.");
+            return make_gate(type, target, control, parameter.real());
+        },
+        "type"_a, "target"_a, "control"_a, "parameter"_a = 0.0);
+    m.def("control_gate", &make_control_gate, "control"_a, "Gate"_a);
+    m.def("prepare_computer_from_circuit", [](int n, const Circuit& c) {
+        auto comp = std::make_shared<Computer>(n);
+        comp->apply_circuit(c);
+        return comp;
+    }, "nqubit"_a, "gates"_a);
+    m.def("prepare_computer_from_gates", [](int n, const std::vector<Gate>& gates) {
+        auto comp = std::make_shared<Computer>(n);
+        for (const auto& g : gates) comp->apply_gate(g);
+        return comp;
+    }, "nqubits"_a, "gates"_a);
+    m.def("add_gate_to_computer", [](const Gate& g, const Computer& c) {
+        auto comp = std::make_shared<Computer>(c);
+        comp->apply_gate(g);
+        return comp;
+    }, "gate"_a, "computer"_a);
+    m.def("inner_product", &qfa::dot, "a"_a, "b"_a);
+    m.def("find_irrep", &find_irrep, py::arg("orb_irrep_to_int"), py::arg("spinorb_indices"));
+    m.def("normalize_angle", &normalize_angle);
+
+    // native exponentiation and Trotterization
+    m.def("exponentiate_pauli_string", &exponentiate_pauli_string, "coefficient"_a, "term"_a, "Use_cRz"_a = false,
+          "ancilla_idx"_a = -1, "Use_open_cRz"_a = false);
+    m.def("trotterize", &trotterize, "operator"_a, "factor"_a = std::complex<double>(1.0, 0.0), "trotter_number"_a = 1,
+          "trotter_order"_a = 1);
+    m.def("trotterize_w_cRz", &trotterize_w_cRz, "operator"_a, "ancilla_qubit_idx"_a,
+          "factor"_a = std::complex<double>(1.0, 0.0), "Use_open_cRz"_a = false, "trotter_number"_a = 1,
+          "trotter_order"_a = 1);
+
+    // mask form of a Pauli circuit (application order)
+    m.def("pauli_masks", [](const Circuit& c) -> py::object {
+        uint64_t x, z;
+        std::complex<double> ph;
+        if (!pauli_masks(c, x, z, ph)) return py::none();
+        return py::make_tuple(x, z, ph);
+    });
+
+    py::class_<UCCProgram, std::shared_ptr<UCCProgram>>(m, "UCCProgram")
+        .def(py::init<int, uint64_t, const std::vector<QubitOperator>&, const std::vector<int>&,
+                      const std::vector<double>&, size_t, const QubitOperator&>(),
+             "nqubit"_a, "ref_index"_a, "generators"_a, "param_index"_a, "scale"_a, "nparam"_a, "hamiltonian"_a)
+        .def("energy", &UCCProgram::energy)
+        .def("gradient", &UCCProgram::gradient)
+        .def("prepare", &UCCProgram::prepare)
+        .def("pool_gradients", &UCCProgram::pool_gradients, "amplitudes"_a, "operators"_a, "batch"_a = 64)
+        .def_property_readonly("nqubit", &UCCProgram::nqubit)
+        .def_property_readonly("nparam", &UCCProgram::nparam);
+
+    py::class_<BatchEngine>(m, "BatchEngine")
+        .def(py::init<uint64_t, int>(), "hbm_budget_bytes"_a, "max_active"_a)
+        .def("submit", &BatchEngine::submit)
+        .def("step", &BatchEngine::step, "cap"_a = 1024)
+        .def("pending", &BatchEngine::pending);
+
+    // engine controls
+    m.def("device_count", []() {
+        int n = 0;
+        qfe_device_count(&n);
+        return n;
+    });
+    m.def("set_tuning", [](int k, int l, int ops) { check(qfe_set_tuning(k, l, ops), "set_tuning"); }, "tile_qubits"_a = 0,
+          "low_qubits"_a = 0, "max_ops_per_pass"_a = 0);
+    m.def("get_tuning", []() {
+        int k, l, o;
+        qfe_get_tuning(&k, &l, &o);
+        return py::make_tuple(k, l, o);
+    });
+    m.def("get_counters", []() {
+        uint64_t f, s, r;
+        double b;
+        qfe_get_counters(&f, &s, &r, &b);
+        return py::dict("fused_passes"_a = f, "single_passes"_a = s, "reduce_passes"_a = r, "bytes"_a = b);
+    });
+    m.def("reset_counters", []() { qfe_reset_counters(); });
+    m.def("synchronize", []() { check(qfe_synchronize(), "synchronize"); });
+    m.def("stream_handle", []() {
+        void* s = nullptr;
+        check(qfe_get_stream(&s), "stream_handle");
+        return (uintptr_t)s;
+    });
+}

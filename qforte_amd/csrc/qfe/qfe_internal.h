@@ -1,0 +1,142 @@
+// Internal declarations shared by the translation units of libqfe.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/qfe.h"
+
+namespace qfe {
+
+// ---- errors ------------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define QFE_HIP(call)                                        \
+    do {                                                     \
+        hipError_t qfe_e_ = (call);                          \
+        if (qfe_e_ != hipSuccess) return ::qfe::hip_fail(qfe_e_, #call); \
+    } while (0)
+
+// ---- library context -----------------------------------------------------------
+struct Tuning {
+    int tile_qubits = 11;  // k: qubits held by one fused tile (2^k amplitudes in LDS)
+    int low_qubits = 4;    // L: lowest qubits always in the tile (contiguous 2^L runs)
+    int max_ops = 24;      // ops per fused pass before LDS traffic dominates
+};
+
+struct Counters {
+    uint64_t fused = 0, single = 0, reduce = 0;
+    double bytes = 0.0;
+};
+
+struct Context {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Tuning tuning;
+    Counters counters;
+    // reusable device scratch for reductions and op descriptors
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // pinned host staging for small results
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+Context& ctx();
+int ensure_init();
+int scratch_reserve(size_t bytes, void** out);
+int pinned_reserve(size_t bytes, void** out);
+
+}  // namespace qfe
+
+struct qfe_state_s {
+    int n = 0;
+    uint64_t N = 0;
+    double2* d = nullptr;
+};
+
+namespace qfe {
+
+// ---- device op descriptors -------------------------------------------------------
+// One operation as the kernels see it.  Every op acts on a set of "mixing"
+// qubits (where it permutes or mixes amplitudes); everything else it needs
+// (controls, diagonal phases, Z masks) is a per-index condition or sign.
+enum OpKind : uint32_t {
+    K_U1 = 0,     // 2x2 unitary on bit t, applied where (j & cmask) == cval
+    K_DIAG1 = 1,  // diagonal 2x2 (d0 on bit t = 0, d1 on bit t = 1) where (j & cmask) == cval
+    K_U2 = 2,     // general 4x4 on bits (t, c), basis 2*c_bit + t_bit
+    K_PEXP = 3,   // exp(i angle P): cos(angle) I + i sin(angle) P, P = phase (-1)^{j.z} |j ^ x>
+};
+
+struct DevOp {
+    uint32_t kind;
+    uint32_t t;       // target bit (global) or local bit inside a fused tile
+    uint32_t c;       // second bit for K_U2
+    uint32_t hb;      // K_PEXP: highest set bit of x (global or local)
+    uint64_t x;       // K_PEXP: flip mask
+    uint64_t z;       // K_PEXP: sign mask (in-tile part when fused)
+    uint64_t cmask;   // condition mask (in-tile part when fused)
+    uint64_t cval;    // condition value
+    uint64_t omask;   // fused only: out-of-tile condition mask (tested on tile base)
+    uint64_t oval;    // fused only: out-of-tile condition value
+    uint64_t ozmask;  // fused only: out-of-tile z mask (sign from tile base)
+    uint32_t tin;     // fused K_DIAG1: 1 if the diagonal bit lies inside the tile
+    uint32_t pad;
+    double2 m[16];    // K_U1: m[0..3] = m00 m01 m10 m11; K_DIAG1: m[0], m[1];
+                      // K_U2: row-major 4x4; K_PEXP: m[0] = (cos, sin), m[1] = phase
+};
+
+// Host-side representation of an op before planning.
+struct HostOp {
+    DevOp op;            // global-bit form
+    uint64_t mix;        // mixing qubits
+    uint64_t support;    // every qubit the op reads (for commutation)
+};
+
+// Build HostOps from gates / op stream. Returns false + message on bad input.
+int lower_gate(const qfe_gate& g, int n, std::vector<HostOp>& out);
+int lower_pexp(const qfe_pauli& p, double angle, int n, std::vector<HostOp>& out);
+
+// Execute a lowered op list on a state (plans fused passes and launches).
+int run_ops(qfe_state_s* s, std::vector<HostOp>& ops);
+
+// ---- reductions / Pauli sums ---------------------------------------------------
+struct PauliGroup {
+    uint64_t x;
+    uint32_t first;  // index into term arrays
+    uint32_t count;
+    uint32_t hb;     // highest bit of x (0 if x == 0)
+    uint32_t pad;
+};
+
+// Group terms by x mask (stable: first-appearance order), folding phase*coeff.
+void group_terms(const qfe_pauli* terms, uint64_t count, std::vector<PauliGroup>& groups,
+                 std::vector<uint64_t>& zs, std::vector<double2>& cs);
+
+int expval_groups(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
+                  const std::vector<uint64_t>& zs, const std::vector<double2>& cs, double2* out_per_group);
+
+uint32_t expval_chunks(int n, size_t ngroups);
+// Asynchronous forms on device-resident group tables: per-group sums land in dout[0..ngroups).
+int launch_expval_async(const double2* bra, const double2* ket, int n, const PauliGroup* dgroups, uint32_t ngroups,
+                        const uint64_t* dzs, const double2* dcs, double2* partial, uint32_t nchunk, double2* dout);
+int launch_apply_sum_async(const double2* in, double2* out, int n, const PauliGroup* dgroups, uint32_t ngroups,
+                           const uint64_t* dzs, const double2* dcs);
+
+// program introspection for the serving engine
+int program_nqubit(qfe_program p);
+uint64_t program_nparam(qfe_program p);
+
+// device helpers exposed across translation units
+int launch_zero(double2* d, uint64_t N, hipStream_t st);
+int launch_basis(double2* d, uint64_t N, uint64_t idx, hipStream_t st);
+int launch_apply_pauli_sum(const double2* in, double2* out, int n, const std::vector<PauliGroup>& groups,
+                           const std::vector<uint64_t>& zs, const std::vector<double2>& cs);
+
+}  // namespace qfe

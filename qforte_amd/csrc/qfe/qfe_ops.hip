@@ -1,0 +1,708 @@
+// Gates and Pauli exponentials: lowering, pass planning and the sweep kernels.
+//
+// The reference applies every gate as its own full-state sweep
+// (computer.cc:340-751) and every Pauli exponential as the 2*n_XY + 2(w-1) + 1
+// gates exponentiate_pauli_string builds (exponentiate.py:9-87).  Here each
+// operation is lowered to one of four device ops (K_U1, K_DIAG1, K_U2,
+// K_PEXP) and a planner groups commuting-safe runs of ops into fused passes:
+// a workgroup loads a tile of 2^k amplitudes spanning a chosen set of qubits
+// into LDS, applies every op of the pass there, and writes the tile back once.
+// A pass holding a single op uses a streaming kernel without LDS.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qfe_common.cuh"
+#include "qfe_internal.h"
+
+namespace qfe {
+
+// ---------------------------------------------------------------------------
+// Lowering (host)
+// ---------------------------------------------------------------------------
+
+static inline double cabs2(const qfe_c64& a) { return a.re * a.re + a.im * a.im; }
+static inline double cabs(const qfe_c64& a) { return std::sqrt(cabs2(a)); }
+static inline bool is_one(const qfe_c64& a) { return a.re == 1.0 && a.im == 0.0; }
+static inline double2 d2(const qfe_c64& a) { return make_double2(a.re, a.im); }
+
+// The reference's element threshold (computer.h:172).
+constexpr double kComputeThreshold = 1.0e-16;
+
+static void add_u1(int t, const qfe_c64 u[4], uint64_t cmask, std::vector<HostOp>& out) {
+    // u = {m00, m01, m10, m11}; classified like computer.cc:353-422
+    const bool diag_nz = cabs(u[0]) + cabs(u[3]) > kComputeThreshold;
+    const bool off_nz = cabs(u[1]) + cabs(u[2]) > kComputeThreshold;
+    HostOp h{};
+    h.op.t = (uint32_t)t;
+    h.op.cmask = cmask;
+    h.op.cval = cmask;
+    h.support = (1ull << t) | cmask;
+    if (diag_nz && !off_nz) {
+        if (is_one(u[0]) && is_one(u[3])) return;  // identity: the reference touches nothing
+        h.op.kind = K_DIAG1;
+        h.op.m[0] = d2(u[0]);
+        h.op.m[1] = d2(u[3]);
+        h.mix = 0;
+    } else {
+        h.op.kind = K_U1;
+        for (int i = 0; i < 4; ++i) h.op.m[i] = d2(u[i]);
+        h.mix = 1ull << t;
+    }
+    out.push_back(h);
+}
+
+int lower_gate(const qfe_gate& g, int n, std::vector<HostOp>& out) {
+    const int t = g.target, c = g.control;
+    if (t < 0 || c < 0) return fail(QFE_ERR_ARG, "apply_gate: negative qubit index");
+    if (t == c) {
+        if (t >= n) return fail(QFE_ERR_ARG, "apply_gate: target qubit " + std::to_string(t) + " out of range for " + std::to_string(n) + " qubits");
+        const qfe_c64 u[4] = {g.m[0], g.m[1], g.m[4], g.m[5]};
+        add_u1(t, u, 0, out);
+        return QFE_OK;
+    }
+    // Controlled-U detection exactly as computer.cc:469-470.
+    const bool controlled = (cabs(g.m[1]) + cabs(g.m[4]) < kComputeThreshold) && is_one(g.m[0]) && is_one(g.m[5]);
+    if (controlled) {
+        if (c >= n) return QFE_OK;  // the control bit reads 0 on every basis state: no-op
+        if (t >= n) return fail(QFE_ERR_ARG, "apply_gate: target qubit " + std::to_string(t) + " out of range for " + std::to_string(n) + " qubits");
+        const qfe_c64 u[4] = {g.m[10], g.m[11], g.m[14], g.m[15]};
+        add_u1(t, u, 1ull << c, out);
+        return QFE_OK;
+    }
+    if (t >= n || c >= n)
+        return fail(QFE_ERR_ARG, "apply_gate: qubit index out of range for a 2-qubit gate on " + std::to_string(n) + " qubits");
+    HostOp h{};
+    h.op.kind = K_U2;
+    h.op.t = (uint32_t)t;
+    h.op.c = (uint32_t)c;
+    for (int i = 0; i < 16; ++i) h.op.m[i] = d2(g.m[i]);
+    h.mix = (1ull << t) | (1ull << c);
+    h.support = h.mix;
+    out.push_back(h);
+    return QFE_OK;
+}
+
+int lower_pexp(const qfe_pauli& p, double angle, int n, std::vector<HostOp>& out) {
+    const uint64_t lim = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    if ((p.x & ~lim) || (p.z & ~lim)) return fail(QFE_ERR_ARG, "apply_pauli_exp: Pauli string acts on a qubit out of range");
+    // Hermitian P needs conj(phase) = phase * (-1)^popcount(x & z).
+    const bool odd = __builtin_popcountll(p.x & p.z) & 1;
+    const double er = odd ? -p.phase.re : p.phase.re, ei = odd ? -p.phase.im : p.phase.im;
+    if (std::fabs(er - p.phase.re) > 1e-15 || std::fabs(-ei - p.phase.im) > 1e-15)
+        return fail(QFE_ERR_ARG, "apply_pauli_exp: the Pauli operator is not Hermitian");
+    if (std::fabs(cabs2(p.phase) - 1.0) > 1e-12) return fail(QFE_ERR_ARG, "apply_pauli_exp: phase must have modulus 1");
+    if (angle == 0.0) return QFE_OK;
+    HostOp h{};
+    h.op.kind = K_PEXP;
+    h.op.x = p.x;
+    h.op.z = p.z;
+    h.op.hb = p.x ? (uint32_t)(63 - __builtin_clzll(p.x)) : 0u;
+    h.op.m[0] = make_double2(std::cos(angle), std::sin(angle));
+    h.op.m[1] = d2(p.phase);
+    h.mix = p.x;
+    h.support = p.x | p.z;
+    out.push_back(h);
+    return QFE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device: single-op streaming kernels
+// ---------------------------------------------------------------------------
+
+constexpr int kTPB = 256;  // threads per block for streaming kernels
+constexpr int kUnroll = 4; // work items per thread
+
+// A 2x2 on (a, b): a' = m00 a + m01 b, b' = m10 a + m11 b.
+__device__ __forceinline__ void u2x2(double2& a, double2& b, const double2* m) {
+    const double2 na = cmix(m[0], a, m[1], b);
+    const double2 nb = cmix(m[2], a, m[3], b);
+    a = na;
+    b = nb;
+}
+
+// exp(i th P) on the pair (a at ja, b at jb = ja ^ x):
+// a' = cos a + i sin ph (-1)^{jb.z} b ; b' = cos b + i sin ph (-1)^{ja.z} a.
+__device__ __forceinline__ void pexp_pair(double2& a, double2& b, double c, double s, double2 ph, bool neg_a, bool neg_b) {
+    const double2 iph = make_double2(-ph.y, ph.x);  // i * phase
+    const double sa = neg_a ? -s : s;               // sign for the term feeding b (from a's index)
+    const double sb = neg_b ? -s : s;               // sign for the term feeding a (from b's index)
+    const double2 ta = cmul(iph, b);
+    const double2 tb = cmul(iph, a);
+    const double2 na = make_double2(c * a.x + sb * ta.x, c * a.y + sb * ta.y);
+    const double2 nb = make_double2(c * b.x + sa * tb.x, c * b.y + sa * tb.y);
+    a = na;
+    b = nb;
+}
+
+// U1 on bit t with an optional single control bit c (cbit = 1<<c or 0).
+__global__ __launch_bounds__(kTPB) void k_u1(double2* __restrict__ s, uint64_t nitems, uint32_t t, uint32_t c,
+                                             uint64_t cbit, double2 m00, double2 m01, double2 m10, double2 m11) {
+    const double2 m[4] = {m00, m01, m10, m11};
+    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint32_t lo = t < c ? t : c, hi = t < c ? c : t;
+    double2 a[kUnroll], b[kUnroll];
+    uint64_t j0[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t k = base + (uint64_t)u * kTPB;
+        if (k < nitems) {
+            j0[u] = cbit ? (ins00(k, lo, hi) | cbit) : ins0(k, t);
+            a[u] = s[j0[u]];
+            b[u] = s[j0[u] | (1ull << t)];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t k = base + (uint64_t)u * kTPB;
+        if (k < nitems) {
+            u2x2(a[u], b[u], m);
+            s[j0[u]] = a[u];
+            s[j0[u] | (1ull << t)] = b[u];
+        }
+    }
+}
+
+// Diagonal on bit t: multiply the bit-t = 0 half by d0 and the bit-t = 1 half
+// by d1, skipping a half whose factor is exactly 1 (computer.cc:370-395).
+// which: 0 = both halves, 1 = only bit=0 half, 2 = only bit=1 half.
+__global__ __launch_bounds__(kTPB) void k_diag1(double2* __restrict__ s, uint64_t nitems, uint32_t t, uint32_t c,
+                                                uint64_t cbit, double2 d0, double2 d1, int which) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint32_t lo = t < c ? t : c, hi = t < c ? c : t;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t k = base + (uint64_t)u * kTPB;
+        if (k < nitems) {
+            const uint64_t j0 = cbit ? (ins00(k, lo, hi) | cbit) : ins0(k, t);
+            if (which != 2) s[j0] = cmul(d0, s[j0]);
+            if (which != 1) {
+                const uint64_t j1 = j0 | (1ull << t);
+                s[j1] = cmul(d1, s[j1]);
+            }
+        }
+    }
+}
+
+// General 4x4 on bits (t, c), basis 2*c_bit + t_bit.
+__global__ __launch_bounds__(kTPB) void k_u2(double2* __restrict__ s, uint64_t nitems, uint32_t t, uint32_t c,
+                                             const double2* __restrict__ mat) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint32_t lo = t < c ? t : c, hi = t < c ? c : t;
+    double2 m[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = mat[i];
+    const uint64_t tb = 1ull << t, cb = 1ull << c;
+#pragma unroll 2
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t k = base + (uint64_t)u * kTPB;
+        if (k < nitems) {
+            const uint64_t j = ins00(k, lo, hi);
+            const uint64_t idx[4] = {j, j | tb, j | cb, j | cb | tb};
+            double2 v[4], r[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = s[idx[q]];
+#pragma unroll
+            for (int row = 0; row < 4; ++row) {
+                double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int col = 0; col < 4; ++col) acc = cadd(acc, cmul(m[row * 4 + col], v[col]));
+                r[row] = acc;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[idx[q]] = r[q];
+        }
+    }
+}
+
+// exp(i th P) with x != 0: pairs (ja, ja ^ x), ja with bit hb clear.
+__global__ __launch_bounds__(kTPB) void k_pexp(double2* __restrict__ s, uint64_t nitems, uint32_t hb, uint64_t x,
+                                               uint64_t z, double c, double sn, double2 ph) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    double2 a[kUnroll], b[kUnroll];
+    uint64_t ja[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t k = base + (uint64_t)u * kTPB;
+        if (k < nitems) {
+            ja[u] = ins0(k, hb);
+            a[u] = s[ja[u]];
+            b[u] = s[ja[u] ^ x];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t k = base + (uint64_t)u * kTPB;
+        if (k < nitems) {
+            const uint64_t jb = ja[u] ^ x;
+            pexp_pair(a[u], b[u], c, sn, ph, __popcll(ja[u] & z) & 1, __popcll(jb & z) & 1);
+            s[ja[u]] = a[u];
+            s[jb] = b[u];
+        }
+    }
+}
+
+// exp(i th P) with x == 0: diagonal phase cos + i sin ph (-1)^{j.z}.
+__global__ __launch_bounds__(kTPB) void k_pexp_diag(double2* __restrict__ s, uint64_t N, uint64_t z, double c,
+                                                    double sn, double2 ph) {
+    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const double2 fp = make_double2(c - sn * ph.y, sn * ph.x);  // cos + i sin ph
+    const double2 fm = make_double2(c + sn * ph.y, -sn * ph.x); // cos - i sin ph
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const uint64_t j = base + (uint64_t)u * kTPB;
+        if (j < N) s[j] = cmul((__popcll(j & z) & 1) ? fm : fp, s[j]);
+    }
+}
+
+static unsigned blocks_for(uint64_t nitems) {
+    const uint64_t per = (uint64_t)kTPB * kUnroll;
+    return (unsigned)((nitems + per - 1) / per);
+}
+
+static int launch_single(qfe_state_s* st, const HostOp& h, double2* dev_mat) {
+    Context& cx = ctx();
+    const DevOp& o = h.op;
+    const uint64_t N = st->N;
+    const uint64_t cbit = o.cmask;  // lowering produces at most one control bit
+    const uint32_t cpos = cbit ? (uint32_t)(63 - __builtin_clzll(cbit)) : o.t;
+    double bytes = 0.0;
+    switch (o.kind) {
+        case K_U1: {
+            const uint64_t items = cbit ? N / 4 : N / 2;
+            hipLaunchKernelGGL(k_u1, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.t, cpos, cbit,
+                               o.m[0], o.m[1], o.m[2], o.m[3]);
+            bytes = 64.0 * (double)items;
+            break;
+        }
+        case K_DIAG1: {
+            const uint64_t items = cbit ? N / 4 : N / 2;
+            const bool one0 = o.m[0].x == 1.0 && o.m[0].y == 0.0;
+            const bool one1 = o.m[1].x == 1.0 && o.m[1].y == 0.0;
+            const int which = one0 ? 2 : (one1 ? 1 : 0);
+            hipLaunchKernelGGL(k_diag1, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.t, cpos, cbit,
+                               o.m[0], o.m[1], which);
+            bytes = (which ? 32.0 : 64.0) * (double)items;
+            break;
+        }
+        case K_U2: {
+            const uint64_t items = N / 4;
+            hipLaunchKernelGGL(k_u2, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.t, o.c,
+                               (const double2*)dev_mat);
+            bytes = 128.0 * (double)items;
+            break;
+        }
+        case K_PEXP: {
+            if (o.x) {
+                const uint64_t items = N / 2;
+                hipLaunchKernelGGL(k_pexp, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.hb, o.x,
+                                   o.z, o.m[0].x, o.m[0].y, o.m[1]);
+                bytes = 64.0 * (double)items;
+            } else {
+                hipLaunchKernelGGL(k_pexp_diag, dim3(blocks_for(N)), dim3(kTPB), 0, cx.stream, st->d, N, o.z, o.m[0].x,
+                                   o.m[0].y, o.m[1]);
+                bytes = 32.0 * (double)N;
+            }
+            break;
+        }
+        default:
+            return fail(QFE_ERR_ARG, "internal: unknown op kind");
+    }
+    QFE_HIP(hipGetLastError());
+    cx.counters.single++;
+    cx.counters.bytes += bytes;
+    return QFE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Device: fused LDS tile pass
+// ---------------------------------------------------------------------------
+
+constexpr int kFusedTPB = 256;
+constexpr int kMaxTileQubits = 13;  // 2^13 * 16 B = 128 KiB of LDS
+
+struct PassDesc {
+    uint64_t bmask;          // tile qubits (global mask)
+    uint32_t k;              // popcount(bmask)
+    uint32_t nlow;           // trailing contiguous tile bits 0..nlow-1
+    uint32_t nops;
+    uint32_t nrest;          // number of non-tile qubits
+    uint8_t hipos[kMaxTileQubits];   // global positions of tile bits nlow..k-1
+    uint8_t restpos[64];             // global positions of non-tile bits, ascending
+};
+
+template <int KQ>
+__global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, const PassDesc* __restrict__ pd,
+                                                     const DevOp* __restrict__ ops) {
+    extern __shared__ double2 tile[];
+    __shared__ uint64_t hioff[1 << 9];
+    constexpr uint32_t T = 1u << KQ;
+    const uint32_t nlow = pd->nlow;
+    const uint32_t nhi = KQ - nlow;
+    // tile base: scatter blockIdx into the non-tile bit positions
+    uint64_t base = 0;
+    {
+        const uint64_t g = blockIdx.x;
+        const uint32_t nr = pd->nrest;
+        for (uint32_t i = 0; i < nr; ++i) base |= ((g >> i) & 1ull) << pd->restpos[i];
+    }
+    // offsets of the high tile bits (2^nhi entries, nhi <= 9)
+    for (uint32_t h = threadIdx.x; h < (1u << nhi); h += kFusedTPB) {
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < nhi; ++i) off |= ((uint64_t)((h >> i) & 1u)) << pd->hipos[i];
+        hioff[h] = off;
+    }
+    __syncthreads();
+    const uint32_t lowmask = (1u << nlow) - 1u;
+    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) tile[l] = s[base | hioff[l >> nlow] | (l & lowmask)];
+    const uint32_t nops = pd->nops;
+    for (uint32_t oi = 0; oi < nops; ++oi) {
+        __syncthreads();
+        const DevOp& o = ops[oi];
+        if ((base & o.omask) != o.oval) continue;  // uniform across the block
+        switch (o.kind) {
+            case K_U1: {
+                const double2 m[4] = {o.m[0], o.m[1], o.m[2], o.m[3]};
+                const uint32_t tb = 1u << o.t;
+                for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                    const uint32_t l0 = (uint32_t)ins0(p, o.t);
+                    if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
+                    double2 a = tile[l0], b = tile[l0 | tb];
+                    u2x2(a, b, m);
+                    tile[l0] = a;
+                    tile[l0 | tb] = b;
+                }
+                break;
+            }
+            case K_DIAG1: {
+                if (o.tin) {
+                    const uint32_t tb = 1u << o.t;
+                    for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                        const uint32_t l0 = (uint32_t)ins0(p, o.t);
+                        if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
+                        tile[l0] = cmul(o.m[0], tile[l0]);
+                        tile[l0 | tb] = cmul(o.m[1], tile[l0 | tb]);
+                    }
+                } else {
+                    const double2 f = ((base >> o.t) & 1ull) ? o.m[1] : o.m[0];
+                    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
+                        if ((l & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
+                        tile[l] = cmul(f, tile[l]);
+                    }
+                }
+                break;
+            }
+            case K_U2: {
+                const uint32_t lo = o.t < o.c ? o.t : o.c, hi = o.t < o.c ? o.c : o.t;
+                const uint32_t tb = 1u << o.t, cb = 1u << o.c;
+                for (uint32_t p = threadIdx.x; p < T / 4; p += kFusedTPB) {
+                    const uint32_t j = (uint32_t)ins00(p, lo, hi);
+                    const uint32_t idx[4] = {j, j | tb, j | cb, j | cb | tb};
+                    double2 v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v[q] = tile[idx[q]];
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) {
+                        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+                        for (int col = 0; col < 4; ++col) acc = cadd(acc, cmul(o.m[row * 4 + col], v[col]));
+                        tile[idx[row]] = acc;
+                    }
+                }
+                break;
+            }
+            case K_PEXP: {
+                const double c = o.m[0].x, sn = o.m[0].y;
+                const double2 ph = o.m[1];
+                const bool bneg = __popcll(base & o.ozmask) & 1;  // sign from out-of-tile z bits
+                const uint32_t zl = (uint32_t)o.z;
+                if (o.x == 0) {
+                    const double2 fp = make_double2(c - sn * ph.y, sn * ph.x);
+                    const double2 fm = make_double2(c + sn * ph.y, -sn * ph.x);
+                    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
+                        const bool neg = ((__popc(l & zl) & 1) != 0) != bneg;
+                        tile[l] = cmul(neg ? fm : fp, tile[l]);
+                    }
+                } else {
+                    const uint32_t xl = (uint32_t)o.x;
+                    for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                        const uint32_t la = (uint32_t)ins0(p, o.hb);
+                        const uint32_t lb = la ^ xl;
+                        double2 a = tile[la], b = tile[lb];
+                        const bool na = ((__popc(la & zl) & 1) != 0) != bneg;
+                        const bool nb = ((__popc(lb & zl) & 1) != 0) != bneg;
+                        pexp_pair(a, b, c, sn, ph, na, nb);
+                        tile[la] = a;
+                        tile[lb] = b;
+                    }
+                }
+                break;
+            }
+            default:
+                break;
+        }
+    }
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) s[base | hioff[l >> nlow] | (l & lowmask)] = tile[l];
+}
+
+// Map a global mask to tile-local bit positions.
+static uint64_t to_local(uint64_t m, const int* local_of) {
+    uint64_t r = 0;
+    while (m) {
+        const int b = __builtin_ctzll(m);
+        m &= m - 1;
+        r |= 1ull << local_of[b];
+    }
+    return r;
+}
+
+// Rewrite a global op for a tile with qubit mask bmask.
+static DevOp localize(const DevOp& g, uint64_t bmask, const int* local_of) {
+    DevOp o = g;
+    o.omask = g.cmask & ~bmask;
+    o.oval = g.cval & ~bmask;
+    o.cmask = to_local(g.cmask & bmask, local_of);
+    o.cval = to_local(g.cval & bmask, local_of);
+    o.ozmask = 0;
+    o.tin = 0;
+    switch (g.kind) {
+        case K_U1:
+            o.t = (uint32_t)local_of[g.t];
+            break;
+        case K_DIAG1:
+            if ((bmask >> g.t) & 1ull) {
+                o.tin = 1;
+                o.t = (uint32_t)local_of[g.t];
+            } else {
+                o.tin = 0;  // o.t stays global: read from the tile base
+            }
+            break;
+        case K_U2:
+            o.t = (uint32_t)local_of[g.t];
+            o.c = (uint32_t)local_of[g.c];
+            break;
+        case K_PEXP:
+            o.x = to_local(g.x, local_of);
+            o.hb = o.x ? (uint32_t)(63 - __builtin_clzll(o.x)) : 0u;
+            o.z = to_local(g.z & bmask, local_of);
+            o.ozmask = g.z & ~bmask;
+            break;
+    }
+    return o;
+}
+
+struct Pass {
+    uint64_t bmask;
+    std::vector<int> ops;  // indices into the lowered op list
+};
+
+// Greedy planner.  Ops may move ahead of an earlier op only when their qubit
+// supports are disjoint (then the two commute exactly).
+static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::vector<Pass>& passes) {
+    const int K = std::min(tu.tile_qubits, n);
+    const int L = std::max(std::min(tu.low_qubits, K), K - 9);
+    const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    const uint64_t lowmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
+    std::vector<char> done(ops.size(), 0);
+    size_t first = 0;
+    while (first < ops.size()) {
+        while (first < ops.size() && done[first]) ++first;
+        if (first >= ops.size()) break;
+        Pass p;
+        p.bmask = lowmask;
+        uint64_t blocked = 0;
+        for (size_t i = first; i < ops.size(); ++i) {
+            if (done[i]) continue;
+            const HostOp& h = ops[i];
+            if (h.support & blocked) {
+                blocked |= h.support;
+                if ((blocked & all) == all) break;
+                continue;
+            }
+            const uint64_t nb = p.bmask | h.mix;
+            if (__builtin_popcountll(nb) <= K && (int)p.ops.size() < tu.max_ops) {
+                p.bmask = nb;
+                p.ops.push_back((int)i);
+                done[i] = 1;
+            } else if (p.ops.empty()) {
+                // does not fit a tile even alone: a standalone streaming pass
+                p.ops.push_back((int)i);
+                done[i] = 1;
+                p.bmask = 0;
+                break;
+            } else {
+                blocked |= h.support;
+                if ((blocked & all) == all) break;
+                if ((int)p.ops.size() >= tu.max_ops) break;
+            }
+        }
+        passes.push_back(std::move(p));
+    }
+}
+
+template <int KQ>
+static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc* pd, const DevOp* ops, hipStream_t st) {
+    const size_t lds = (size_t)sizeof(double2) << KQ;
+    static bool attr_set = false;  // large tiles need the dynamic-LDS limit raised once
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fused<KQ>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd, ops);
+}
+
+static int launch_fused(qfe_state_s* st, int k, const PassDesc* dpd, const DevOp* dops) {
+    Context& cx = ctx();
+    const uint64_t nblocks = st->N >> k;
+    switch (k) {
+        case 2: launch_fused_k<2>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 3: launch_fused_k<3>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 4: launch_fused_k<4>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 5: launch_fused_k<5>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 6: launch_fused_k<6>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 7: launch_fused_k<7>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 8: launch_fused_k<8>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 9: launch_fused_k<9>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 10: launch_fused_k<10>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 11: launch_fused_k<11>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 12: launch_fused_k<12>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 13: launch_fused_k<13>(st->d, nblocks, dpd, dops, cx.stream); break;
+        default: return fail(QFE_ERR_ARG, "internal: fused tile size out of range");
+    }
+    QFE_HIP(hipGetLastError());
+    cx.counters.fused++;
+    cx.counters.bytes += 32.0 * (double)st->N;
+    return QFE_OK;
+}
+
+int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
+    if (ops.empty()) return QFE_OK;
+    Context& cx = ctx();
+    const int n = st->n;
+    std::vector<Pass> passes;
+    plan(ops, n, cx.tuning, passes);
+
+    // Build all device descriptors for this call in one host buffer, upload once.
+    // Layout: [PassDesc per fused pass][DevOp arrays][U2 matrices for single passes]
+    std::vector<PassDesc> pdescs;
+    std::vector<DevOp> dops;
+    std::vector<size_t> pass_pd(passes.size(), (size_t)-1), pass_op0(passes.size(), 0);
+    std::vector<size_t> single_mat(passes.size(), (size_t)-1);
+    std::vector<double2> mats;
+    for (size_t pi = 0; pi < passes.size(); ++pi) {
+        const Pass& p = passes[pi];
+        if (p.ops.size() == 1) {
+            const HostOp& h = ops[p.ops[0]];
+            if (h.op.kind == K_U2) {
+                single_mat[pi] = mats.size();
+                for (int i = 0; i < 16; ++i) mats.push_back(h.op.m[i]);
+            }
+            continue;
+        }
+        // extend the tile with the lowest free qubits up to K bits
+        const int K = std::min(cx.tuning.tile_qubits, n);
+        uint64_t b = p.bmask;
+        for (int q = 0; q < n && __builtin_popcountll(b) < K; ++q) b |= 1ull << q;
+        PassDesc pd{};
+        pd.bmask = b;
+        pd.k = (uint32_t)__builtin_popcountll(b);
+        uint32_t nlow = 0;
+        while (nlow < 64 && ((b >> nlow) & 1ull)) ++nlow;
+        // the planner keeps at least k - 9 low bits in every tile, so the
+        // offset table of the high tile bits never exceeds 2^9 entries
+        if (pd.k - nlow > 9) return fail(QFE_ERR_STATE, "internal: fused tile has more than 9 high bits");
+        pd.nlow = nlow;
+        int local_of[64];
+        for (int q = 0; q < 64; ++q) local_of[q] = -1;
+        int li = 0, hi_i = 0, ri = 0;
+        for (int q = 0; q < n; ++q) {
+            if ((b >> q) & 1ull) {
+                if ((uint32_t)li >= nlow) pd.hipos[hi_i++] = (uint8_t)q;
+                local_of[q] = li++;
+            } else {
+                pd.restpos[ri++] = (uint8_t)q;
+            }
+        }
+        pd.nrest = (uint32_t)ri;
+        pd.nops = (uint32_t)p.ops.size();
+        pass_pd[pi] = pdescs.size();
+        pdescs.push_back(pd);
+        pass_op0[pi] = dops.size();
+        for (int oi : p.ops) dops.push_back(localize(ops[oi].op, b, local_of));
+    }
+    const size_t bytes_pd = pdescs.size() * sizeof(PassDesc);
+    const size_t off_ops = (bytes_pd + 255) & ~(size_t)255;
+    const size_t bytes_ops = dops.size() * sizeof(DevOp);
+    const size_t off_mat = (off_ops + bytes_ops + 255) & ~(size_t)255;
+    const size_t total = off_mat + mats.size() * sizeof(double2);
+    char* dev = nullptr;
+    if (total) {
+        void* scr = nullptr;
+        int rc = scratch_reserve(total, &scr);
+        if (rc) return rc;
+        dev = static_cast<char*>(scr);
+        std::vector<char> host(total);
+        if (bytes_pd) std::memcpy(host.data(), pdescs.data(), bytes_pd);
+        if (bytes_ops) std::memcpy(host.data() + off_ops, dops.data(), bytes_ops);
+        if (!mats.empty()) std::memcpy(host.data() + off_mat, mats.data(), mats.size() * sizeof(double2));
+        QFE_HIP(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, cx.stream));
+        // the host staging buffer dies at scope exit: make sure the copy consumed it
+        QFE_HIP(hipStreamSynchronize(cx.stream));
+    }
+    for (size_t pi = 0; pi < passes.size(); ++pi) {
+        const Pass& p = passes[pi];
+        int rc;
+        if (p.ops.size() == 1) {
+            double2* m = single_mat[pi] == (size_t)-1 ? nullptr : reinterpret_cast<double2*>(dev + off_mat) + single_mat[pi];
+            rc = launch_single(st, ops[p.ops[0]], m);
+        } else {
+            const PassDesc* dpd = reinterpret_cast<const PassDesc*>(dev) + pass_pd[pi];
+            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops) + pass_op0[pi];
+            rc = launch_fused(st, (int)pdescs[pass_pd[pi]].k, dpd, dop);
+        }
+        if (rc) return rc;
+    }
+    return QFE_OK;
+}
+
+}  // namespace qfe
+
+using namespace qfe;
+
+extern "C" {
+
+int qfe_apply_gates(qfe_state s, const qfe_gate* gates, uint64_t count) {
+    if (!s || !s->d || (count && !gates)) return fail(QFE_ERR_ARG, "qfe_apply_gates: invalid argument");
+    std::vector<HostOp> ops;
+    ops.reserve(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        int rc = lower_gate(gates[i], s->n, ops);
+        if (rc) return rc;
+    }
+    return run_ops(s, ops);
+}
+
+int qfe_apply_ops(qfe_state s, const qfe_op* ops_in, uint64_t count) {
+    if (!s || !s->d || (count && !ops_in)) return fail(QFE_ERR_ARG, "qfe_apply_ops: invalid argument");
+    std::vector<HostOp> ops;
+    ops.reserve(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        int rc;
+        if (ops_in[i].kind == QFE_OP_GATE)
+            rc = lower_gate(ops_in[i].gate, s->n, ops);
+        else if (ops_in[i].kind == QFE_OP_PEXP)
+            rc = lower_pexp(ops_in[i].pauli, ops_in[i].angle, s->n, ops);
+        else
+            rc = fail(QFE_ERR_ARG, "qfe_apply_ops: unknown op kind");
+        if (rc) return rc;
+    }
+    return run_ops(s, ops);
+}
+
+}  // extern "C"

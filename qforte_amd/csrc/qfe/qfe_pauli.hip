@@ -1,0 +1,372 @@
+// Pauli sums: expectation values, cross expectation values and apply_operator.
+//
+// The reference evaluates <psi|H|psi> by copying psi, applying every term's
+// Pauli gates as separate sweeps, scaling, accumulating and restoring the copy
+// (computer.cc:80-91, 753-771): (w + 4) sweeps per term.  Here terms are
+// grouped by X mask; every term of a group pairs the same amplitudes
+// (j, j ^ x), so one read pass over the state evaluates a whole group:
+//   sum_{t in g} c_t <bra|P_t|ket> = sum_{pairs} conj(bra_a) ket_b f(b) + conj(bra_b) ket_a f(a),
+//   f(j) = sum_{t in g} c_t phase_t (-1)^popcount(j & z_t).
+// Reductions are two-stage with a fixed order, so results are bitwise
+// reproducible run to run and independent of what else runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "qfe_common.cuh"
+#include "qfe_internal.h"
+
+namespace qfe {
+
+void group_terms(const qfe_pauli* terms, uint64_t count, std::vector<PauliGroup>& groups, std::vector<uint64_t>& zs,
+                 std::vector<double2>& cs) {
+    groups.clear();
+    zs.clear();
+    cs.clear();
+    std::unordered_map<uint64_t, size_t> where;
+    std::vector<std::vector<uint64_t>> members;
+    for (uint64_t i = 0; i < count; ++i) {
+        auto it = where.find(terms[i].x);
+        if (it == where.end()) {
+            where.emplace(terms[i].x, members.size());
+            members.emplace_back();
+            members.back().push_back(i);
+        } else {
+            members[it->second].push_back(i);
+        }
+    }
+    for (auto& mem : members) {
+        PauliGroup g{};
+        g.x = terms[mem[0]].x;
+        g.first = (uint32_t)zs.size();
+        g.count = (uint32_t)mem.size();
+        g.hb = g.x ? (uint32_t)(63 - __builtin_clzll(g.x)) : 0u;
+        for (uint64_t i : mem) {
+            const qfe_pauli& t = terms[i];
+            zs.push_back(t.z);
+            // fold the coefficient and the phase of the mask form
+            cs.push_back(make_double2(t.coeff.re * t.phase.re - t.coeff.im * t.phase.im,
+                                      t.coeff.re * t.phase.im + t.coeff.im * t.phase.re));
+        }
+        groups.push_back(g);
+    }
+}
+
+constexpr int kEvTPB = 256;
+
+// f(j) = sum_t c_t (-1)^popcount(j & z_t)
+__device__ __forceinline__ double2 group_f(uint64_t j, const uint64_t* __restrict__ zs, const double2* __restrict__ cs,
+                                           uint32_t count) {
+    double2 f = make_double2(0.0, 0.0);
+    for (uint32_t t = 0; t < count; ++t) {
+        const double2 c = cs[t];
+        if (__popcll(j & zs[t]) & 1) {
+            f.x -= c.x;
+            f.y -= c.y;
+        } else {
+            f.x += c.x;
+            f.y += c.y;
+        }
+    }
+    return f;
+}
+
+// One block = (group g, chunk ch).  Pairs of group g are split into nchunk
+// equal contiguous ranges.
+__global__ __launch_bounds__(kEvTPB) void k_expval_groups(const double2* __restrict__ bra,
+                                                          const double2* __restrict__ ket, int n,
+                                                          const PauliGroup* __restrict__ groups,
+                                                          const uint64_t* __restrict__ zs,
+                                                          const double2* __restrict__ cs, uint32_t nchunk,
+                                                          double2* __restrict__ partial) {
+    __shared__ double2 lds[16];
+    const uint32_t g = blockIdx.x / nchunk;
+    const uint32_t ch = blockIdx.x % nchunk;
+    const PauliGroup G = groups[g];
+    const uint64_t* z = zs + G.first;
+    const double2* c = cs + G.first;
+    double2 acc = make_double2(0.0, 0.0);
+    if (G.x == 0) {
+        const uint64_t N = 1ull << n;
+        const uint64_t per = (N + nchunk - 1) / nchunk;
+        const uint64_t b0 = (uint64_t)ch * per;
+        const uint64_t e0 = b0 + per < N ? b0 + per : N;
+        for (uint64_t j = b0 + threadIdx.x; j < e0; j += kEvTPB) {
+            const double2 p = cconjmul(bra[j], ket[j]);
+            acc = cadd(acc, cmul(p, group_f(j, z, c, G.count)));
+        }
+    } else {
+        const uint64_t P = 1ull << (n - 1);
+        const uint64_t per = (P + nchunk - 1) / nchunk;
+        const uint64_t b0 = (uint64_t)ch * per;
+        const uint64_t e0 = b0 + per < P ? b0 + per : P;
+        for (uint64_t k = b0 + threadIdx.x; k < e0; k += kEvTPB) {
+            const uint64_t ja = ins0(k, G.hb);
+            const uint64_t jb = ja ^ G.x;
+            const double2 ab = cconjmul(bra[ja], ket[jb]);
+            const double2 ba = cconjmul(bra[jb], ket[ja]);
+            acc = cadd(acc, cmul(ab, group_f(jb, z, c, G.count)));
+            acc = cadd(acc, cmul(ba, group_f(ja, z, c, G.count)));
+        }
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kEvTPB) void k_sum_groups(const double2* __restrict__ partial, uint32_t nchunk,
+                                                       double2* __restrict__ out) {
+    __shared__ double2 lds[16];
+    const double2* row = partial + (uint64_t)blockIdx.x * nchunk;
+    double2 acc = make_double2(0.0, 0.0);
+    for (uint32_t i = threadIdx.x; i < nchunk; i += kEvTPB) acc = cadd(acc, row[i]);
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc;
+}
+
+__global__ void k_apply_sum(const double2* __restrict__ in, double2* __restrict__ out, uint64_t N,
+                            const PauliGroup* __restrict__ groups, uint32_t ngroups, const uint64_t* __restrict__ zs,
+                            const double2* __restrict__ cs);
+
+// Upload groups/terms into device scratch after `reserved` bytes; returns pointers.
+struct GroupDev {
+    PauliGroup* groups;
+    uint64_t* zs;
+    double2* cs;
+    char* tail;  // first free byte after the arrays
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int upload_groups(const std::vector<PauliGroup>& groups, const std::vector<uint64_t>& zs,
+                         const std::vector<double2>& cs, size_t extra, GroupDev& out) {
+    const size_t b_g = align256(groups.size() * sizeof(PauliGroup));
+    const size_t b_z = align256(zs.size() * sizeof(uint64_t));
+    const size_t b_c = align256(cs.size() * sizeof(double2));
+    void* scr = nullptr;
+    int rc = scratch_reserve(b_g + b_z + b_c + extra, &scr);
+    if (rc) return rc;
+    char* base = static_cast<char*>(scr);
+    std::vector<char> host(b_g + b_z + b_c);
+    std::memcpy(host.data(), groups.data(), groups.size() * sizeof(PauliGroup));
+    std::memcpy(host.data() + b_g, zs.data(), zs.size() * sizeof(uint64_t));
+    std::memcpy(host.data() + b_g + b_z, cs.data(), cs.size() * sizeof(double2));
+    QFE_HIP(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, ctx().stream));
+    QFE_HIP(hipStreamSynchronize(ctx().stream));
+    out.groups = reinterpret_cast<PauliGroup*>(base);
+    out.zs = reinterpret_cast<uint64_t*>(base + b_g);
+    out.cs = reinterpret_cast<double2*>(base + b_g + b_z);
+    out.tail = base + b_g + b_z + b_c;
+    return QFE_OK;
+}
+
+static uint32_t chunks_for(int n, size_t ngroups) {
+    // aim for ~4096 pairs per block, at most 1024 chunks per group, and keep
+    // the partials array bounded
+    const uint64_t P = 1ull << (n > 0 ? n - 1 : 0);
+    uint64_t ch = (P + 4095) / 4096;
+    if (ch > 1024) ch = 1024;
+    if (ch < 1) ch = 1;
+    while (ch > 1 && ch * ngroups > (1ull << 22)) ch >>= 1;
+    return (uint32_t)ch;
+}
+
+uint32_t expval_chunks(int n, size_t ngroups) { return chunks_for(n, ngroups); }
+
+int launch_expval_async(const double2* bra, const double2* ket, int n, const PauliGroup* dgroups, uint32_t ngroups,
+                        const uint64_t* dzs, const double2* dcs, double2* partial, uint32_t nchunk,
+                        double2* dout) {
+    Context& cx = ctx();
+    if (ngroups == 0) return QFE_OK;
+    hipLaunchKernelGGL(k_expval_groups, dim3((unsigned)((uint64_t)ngroups * nchunk)), dim3(kEvTPB), 0, cx.stream, bra,
+                       ket, n, dgroups, dzs, dcs, nchunk, partial);
+    QFE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_groups, dim3(ngroups), dim3(kEvTPB), 0, cx.stream, partial, nchunk, dout);
+    QFE_HIP(hipGetLastError());
+    cx.counters.reduce += ngroups;
+    cx.counters.bytes += (bra == ket ? 16.0 : 32.0) * (double)(1ull << n) * (double)ngroups;
+    return QFE_OK;
+}
+
+int launch_apply_sum_async(const double2* in, double2* out, int n, const PauliGroup* dgroups, uint32_t ngroups,
+                           const uint64_t* dzs, const double2* dcs) {
+    Context& cx = ctx();
+    const uint64_t N = 1ull << n;
+    if (ngroups == 0) return launch_zero(out, N, cx.stream);
+    hipLaunchKernelGGL(k_apply_sum, dim3((unsigned)((N + kEvTPB - 1) / kEvTPB)), dim3(kEvTPB), 0, cx.stream, in, out,
+                       N, dgroups, ngroups, dzs, dcs);
+    QFE_HIP(hipGetLastError());
+    cx.counters.single++;
+    cx.counters.bytes += 16.0 * (double)N * (double)(ngroups + 1);
+    return QFE_OK;
+}
+
+int expval_groups(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
+                  const std::vector<uint64_t>& zs, const std::vector<double2>& cs, double2* out_per_group) {
+    Context& cx = ctx();
+    const size_t G = groups.size();
+    if (G == 0) return QFE_OK;
+    const uint32_t nchunk = chunks_for(n, G);
+    const size_t b_part = align256(G * nchunk * sizeof(double2));
+    const size_t b_out = align256(G * sizeof(double2));
+    GroupDev gd;
+    int rc = upload_groups(groups, zs, cs, b_part + b_out, gd);
+    if (rc) return rc;
+    double2* partial = reinterpret_cast<double2*>(gd.tail);
+    double2* dout = reinterpret_cast<double2*>(gd.tail + b_part);
+    // launch in slices to keep the grid below 2^31 blocks
+    const uint64_t total_blocks = (uint64_t)G * nchunk;
+    hipLaunchKernelGGL(k_expval_groups, dim3((unsigned)total_blocks), dim3(kEvTPB), 0, cx.stream, bra, ket, n,
+                       gd.groups, gd.zs, gd.cs, nchunk, partial);
+    QFE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)G), dim3(kEvTPB), 0, cx.stream, partial, nchunk, dout);
+    QFE_HIP(hipGetLastError());
+    void* pin = nullptr;
+    rc = pinned_reserve(G * sizeof(double2), &pin);
+    if (rc) return rc;
+    QFE_HIP(hipMemcpyAsync(pin, dout, G * sizeof(double2), hipMemcpyDeviceToHost, cx.stream));
+    QFE_HIP(hipStreamSynchronize(cx.stream));
+    std::memcpy(out_per_group, pin, G * sizeof(double2));
+    cx.counters.reduce += G;
+    const double per = (bra == ket ? 16.0 : 32.0) * (double)(1ull << n);
+    cx.counters.bytes += per * (double)G;
+    return QFE_OK;
+}
+
+// out_k = sum_g ket[k ^ x_g] f_g(k ^ x_g): every thread owns output amplitudes
+// and walks the groups in order (fixed summation order per element).
+__global__ void k_apply_sum(const double2* __restrict__ in, double2* __restrict__ out, uint64_t N,
+                            const PauliGroup* __restrict__ groups, uint32_t ngroups, const uint64_t* __restrict__ zs,
+                            const double2* __restrict__ cs) {
+    const uint64_t k = (uint64_t)blockIdx.x * kEvTPB + threadIdx.x;
+    if (k >= N) return;
+    double2 acc = make_double2(0.0, 0.0);
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        const PauliGroup G = groups[g];
+        const uint64_t src = k ^ G.x;
+        const double2 f = group_f(src, zs + G.first, cs + G.first, G.count);
+        acc = cadd(acc, cmul(f, in[src]));
+    }
+    out[k] = acc;
+}
+
+int launch_apply_pauli_sum(const double2* in, double2* out, int n, const std::vector<PauliGroup>& groups,
+                           const std::vector<uint64_t>& zs, const std::vector<double2>& cs) {
+    Context& cx = ctx();
+    const uint64_t N = 1ull << n;
+    if (groups.empty()) return launch_zero(out, N, cx.stream);
+    GroupDev gd;
+    int rc = upload_groups(groups, zs, cs, 0, gd);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_apply_sum, dim3((unsigned)((N + kEvTPB - 1) / kEvTPB)), dim3(kEvTPB), 0, cx.stream, in, out,
+                       N, gd.groups, (uint32_t)groups.size(), gd.zs, gd.cs);
+    QFE_HIP(hipGetLastError());
+    cx.counters.single++;
+    cx.counters.bytes += 16.0 * (double)N * (double)(groups.size() + 1);
+    return QFE_OK;
+}
+
+static bool valid(qfe_state s) { return s != nullptr && s->d != nullptr; }
+
+static int check_terms(const qfe_pauli* terms, uint64_t count, int n) {
+    if (count && !terms) return fail(QFE_ERR_ARG, "Pauli terms: null pointer");
+    const uint64_t lim = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+    for (uint64_t i = 0; i < count; ++i)
+        if ((terms[i].x & ~lim) || (terms[i].z & ~lim))
+            return fail(QFE_ERR_ARG, "Pauli term acts on a qubit outside the register");
+    return QFE_OK;
+}
+
+}  // namespace qfe
+
+using namespace qfe;
+
+extern "C" {
+
+int qfe_expval(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out) {
+    if (!valid(s) || !out) return fail(QFE_ERR_ARG, "qfe_expval: invalid argument");
+    int rc = check_terms(terms, count, s->n);
+    if (rc) return rc;
+    std::vector<PauliGroup> groups;
+    std::vector<uint64_t> zs;
+    std::vector<double2> cs;
+    group_terms(terms, count, groups, zs, cs);
+    std::vector<double2> per(groups.size());
+    rc = expval_groups(s->d, s->d, s->n, groups, zs, cs, per.data());
+    if (rc) return rc;
+    double2 tot = make_double2(0.0, 0.0);
+    for (const double2& v : per) tot = make_double2(tot.x + v.x, tot.y + v.y);
+    out->re = tot.x;
+    out->im = tot.y;
+    return QFE_OK;
+}
+
+int qfe_expval_many(qfe_state s, const qfe_pauli* terms, const uint64_t* op_offsets, uint64_t nops, qfe_c64* out) {
+    if (!valid(s) || (nops && (!op_offsets || !out))) return fail(QFE_ERR_ARG, "qfe_expval_many: invalid argument");
+    if (nops == 0) return QFE_OK;
+    const uint64_t total = op_offsets[nops];
+    int rc = check_terms(terms, total, s->n);
+    if (rc) return rc;
+    // one launch for all operators: groups never span operators
+    std::vector<PauliGroup> groups, g1;
+    std::vector<uint64_t> zs, z1;
+    std::vector<double2> cs, c1;
+    std::vector<uint64_t> op_of_group;
+    for (uint64_t k = 0; k < nops; ++k) {
+        if (op_offsets[k + 1] < op_offsets[k]) return fail(QFE_ERR_ARG, "qfe_expval_many: offsets must be non-decreasing");
+        group_terms(terms + op_offsets[k], op_offsets[k + 1] - op_offsets[k], g1, z1, c1);
+        const uint32_t shift = (uint32_t)zs.size();
+        for (PauliGroup g : g1) {
+            g.first += shift;
+            groups.push_back(g);
+            op_of_group.push_back(k);
+        }
+        zs.insert(zs.end(), z1.begin(), z1.end());
+        cs.insert(cs.end(), c1.begin(), c1.end());
+    }
+    std::vector<double2> per(groups.size());
+    rc = expval_groups(s->d, s->d, s->n, groups, zs, cs, per.data());
+    if (rc) return rc;
+    for (uint64_t k = 0; k < nops; ++k) out[k] = qfe_c64{0.0, 0.0};
+    for (size_t g = 0; g < groups.size(); ++g) {
+        out[op_of_group[g]].re += per[g].x;
+        out[op_of_group[g]].im += per[g].y;
+    }
+    return QFE_OK;
+}
+
+int qfe_cross_expval(qfe_state bra, qfe_state ket, const qfe_pauli* terms, uint64_t count, qfe_c64* out) {
+    if (!valid(bra) || !valid(ket) || !out) return fail(QFE_ERR_ARG, "qfe_cross_expval: invalid argument");
+    if (bra->n != ket->n) return fail(QFE_ERR_ARG, "qfe_cross_expval: qubit counts differ");
+    int rc = check_terms(terms, count, bra->n);
+    if (rc) return rc;
+    std::vector<PauliGroup> groups;
+    std::vector<uint64_t> zs;
+    std::vector<double2> cs;
+    group_terms(terms, count, groups, zs, cs);
+    std::vector<double2> per(groups.size());
+    rc = expval_groups(bra->d, ket->d, bra->n, groups, zs, cs, per.data());
+    if (rc) return rc;
+    double2 tot = make_double2(0.0, 0.0);
+    for (const double2& v : per) tot = make_double2(tot.x + v.x, tot.y + v.y);
+    out->re = tot.x;
+    out->im = tot.y;
+    return QFE_OK;
+}
+
+int qfe_apply_pauli_sum(qfe_state in, const qfe_pauli* terms, uint64_t count, qfe_state out) {
+    if (!valid(in) || !valid(out)) return fail(QFE_ERR_ARG, "qfe_apply_pauli_sum: invalid handle");
+    if (in == out) return fail(QFE_ERR_ARG, "qfe_apply_pauli_sum: output must differ from input");
+    if (in->n != out->n) return fail(QFE_ERR_ARG, "qfe_apply_pauli_sum: qubit counts differ");
+    int rc = check_terms(terms, count, in->n);
+    if (rc) return rc;
+    std::vector<PauliGroup> groups;
+    std::vector<uint64_t> zs;
+    std::vector<double2> cs;
+    group_terms(terms, count, groups, zs, cs);
+    return launch_apply_pauli_sum(in->d, out->d, in->n, groups, zs, cs);
+}
+
+}  // extern "C"

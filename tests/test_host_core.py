@@ -1,0 +1,245 @@
+"""The native host core against the compiled reference core and the
+reference's own known answers: gates, circuits, Pauli algebra, JW, pools."""
+
+import math
+
+import numpy as np
+import pytest
+
+from qforte_amd.engine import workloads as W
+
+
+def terms_of(op):
+    return [(t[0], t[1].str()) for t in op.terms()]
+
+
+# ---- gates (tests/test_gates.py:258-459) ------------------------------------------------
+
+def test_gate_parameters_and_adjoints(qf):
+    g = qf.gate("X", 0)
+    assert g.nqubits() == 1 and g.gate_id() == "X" and g.has_parameter() is False and g.parameter() is None
+    with pytest.raises(ValueError):
+        g.update_parameter(1.0)
+    r = qf.gate("R", 0, 0.5)
+    assert str(r.adjoint()) == "R0" and str(r.adjoint().adjoint()) == "R0"
+    g2 = qf.gate("cR", 0, 1, 1.0).update_parameter(2.0)
+    assert g2.gate_id() == "cR" and g2.parameter() == pytest.approx(2.0)
+    assert qf.gate("X", 0) == qf.gate("X", 0) and qf.gate("X", 0) != qf.gate("X", 1)
+    assert qf.gate("R", 0, 0.5).adjoint() == qf.gate("R", 0, -0.5)
+    for name in ["X", "Y", "Z", "H", "I"]:
+        assert qf.gate(name, 0).adjoint() == qf.gate(name, 0)
+    for name in ["V", "S", "T"]:
+        assert qf.gate(name, 0).adjoint() != qf.gate(name, 0)
+    assert qf.gate("T", 0).adjoint().gate_id() == "R" and qf.gate("T", 0).adjoint().parameter() == -np.pi / 4
+    assert qf.gate("S", 0).adjoint().parameter() == -np.pi / 2
+    for name in ["cX", "CNOT", "acX", "aCNOT", "SWAP", "cY", "cZ"]:
+        assert qf.gate(name, 0, 1).adjoint() == qf.gate(name, 0, 1)
+    assert qf.gate("cV", 0, 1).adjoint() != qf.gate("cV", 0, 1)
+    for name in ["cR", "cRz"]:
+        assert qf.gate(name, 0, 1, 0.5).adjoint() == qf.gate(name, 0, 1, -0.5)
+    assert qf.gate("A", 0, 1, 0.7).adjoint().parameter() == pytest.approx(0.7)
+    assert qf.gate("R", 0, 5 * np.pi / 2).parameter() == pytest.approx(np.pi / 2, abs=1e-16)
+    assert qf.gate("Rx", 0, -9 * np.pi / 2).parameter() == pytest.approx(-np.pi / 2, abs=1e-16)
+    assert qf.gate("A", 0, 1, 5 * np.pi / 2).parameter() == pytest.approx(np.pi / 2, abs=1e-16)
+    assert qf.gate("cRz", 0, 1, -9 * np.pi / 2).parameter() == pytest.approx(-np.pi / 2, abs=1e-16)
+    with pytest.raises(ValueError):
+        qf.gate("CNOT", 0, 1.0)
+    with pytest.raises(ValueError):
+        qf.gate("foo", 0, 1, 0.3)
+
+
+@pytest.mark.parametrize("name,p", [("X", 0), ("Y", 0), ("Z", 0), ("H", 0), ("S", 0), ("T", 0), ("V", 0),
+                                    ("R", 0.7), ("Rx", 1.3), ("Ry", -2.1), ("Rz", 5.9), ("cX", 0), ("acX", 0),
+                                    ("cY", 0), ("cZ", 0), ("cV", 0), ("SWAP", 0), ("cR", 0.9), ("cRz", -3.3),
+                                    ("A", 1.1)])
+def test_gate_strings_and_adjoints_match_reference(qf, ref, name, p):
+    two = name in ("cX", "acX", "cY", "cZ", "cV", "SWAP", "cR", "cRz", "A")
+    t, c = (2, 5) if two else (3, 3)
+    a = qf.gate(name, t, c, p) if p else qf.gate(name, t, c)
+    b = ref.gate(name, t, c, p) if p else ref.gate(name, t, c)
+    assert str(a) == str(b) and repr(a) == repr(b)
+    assert str(a.adjoint()) == str(b.adjoint()) and repr(a.adjoint()) == repr(b.adjoint())
+    assert a.adjoint().parameter() == b.adjoint().parameter()
+
+
+# ---- circuits (tests/test_circuit.py:50-206) ----------------------------------------------
+
+def test_circuit_editing_and_errors(qf):
+    c = qf.build_circuit("H_0 X_1 Y_2 Z_3")
+    assert str(c) == "[Z3 Y2 X1 H0]"
+    c.insert_gate(1, qf.gate("S", 5))
+    assert str(c) == "[Z3 Y2 X1 S5 H0]"
+    c.remove_gate(1)
+    c.swap_gates(0, 3)
+    assert str(c) == "[H0 Y2 X1 Z3]"
+    c.replace_gate(0, qf.gate("T", 4))
+    assert str(c) == "[H0 Y2 X1 T4]"
+    for fn, args, msg in [
+        (c.insert_gate, (10, qf.gate("X", 0)), "Circuit::insert_gate: position out of range"),
+        (c.remove_gate, (10,), "Circuit::remove_gate: position out of range"),
+        (c.replace_gate, (10, qf.gate("X", 0)), "Circuit::replace_gate: position out of range"),
+        (c.swap_gates, (0, 10), "Circuit::swap_gates: position out of range"),
+        (c.insert_circuit, (10, qf.Circuit()), "Circuit::insert_circuit: position out of range"),
+        (c.remove_gates, (0, 10), "Circuit::remove_gates: position out of range"),
+    ]:
+        with pytest.raises(RuntimeError, match=msg.replace("(", r"\(")):
+            fn(*args)
+    p = qf.build_circuit("Rx_0_0.1 Ry_1_0.2 cX_0_1 Rz_2_0.3")
+    assert p.get_parameters() == pytest.approx([0.1, 0.2, 0.3])
+    p.set_parameters([1.0, 2.0, 3.0])
+    assert p.get_parameters() == pytest.approx([1.0, 2.0, 3.0])
+    with pytest.raises(RuntimeError, match="number of parameters"):
+        p.set_parameters([1.0])
+    assert p.get_num_cnots() == 1
+    two = qf.build_circuit("SWAP_0_1")
+    two.add(qf.gate("A", 1, 0, 0.2))
+    assert two.get_num_cnots() == 6
+
+
+def test_pauli_canonicalisation_matches_reference(qf, ref):
+    r = np.random.default_rng(3)
+    for trial in range(200):
+        w = int(r.integers(1, 9))
+        word = " ".join(f"{'XYZ'[int(r.integers(3))]}_{int(r.integers(5))}" for _ in range(w))
+        a, b = qf.build_circuit(word), ref.build_circuit(word) if hasattr(ref, "build_circuit") else None
+        b = ref.Circuit()
+        for tok in word.split():
+            l, q = tok.split("_")
+            b.add(ref.gate(l, int(q)))
+        ca, cb = a.canonicalize_pauli_circuit(), b.canonicalize_pauli_circuit()
+        assert ca == cb and str(a) == str(b)
+
+
+def test_qubit_operator_algebra_matches_reference(qf, ref):
+    for seed in range(6):
+        t1 = W.random_pauli_sum(6, 12, seed=seed, real_coeffs=False)
+        t2 = W.random_pauli_sum(6, 9, seed=seed + 50, real_coeffs=False)
+        a1, a2 = W.to_operator(qf, t1), W.to_operator(qf, t2)
+        b1, b2 = W.to_operator(ref, t1), W.to_operator(ref, t2)
+        a1.operator_product(a2, True, True)
+        b1.operator_product(b2, True, True)
+        assert terms_of(a1) == terms_of(b1)
+        a1.order_terms()
+        b1.order_terms()
+        assert terms_of(a1) == terms_of(b1)
+        assert a1.check_op_equivalence(a1, True)
+        a3, b3 = W.to_operator(qf, t1), W.to_operator(ref, t1)
+        a3.simplify(False)
+        b3.simplify(False)
+        assert terms_of(a3) == terms_of(b3)
+        assert a3.str() == b3.str()
+
+
+# ---- JW (tests/test_jw_transform.py) -------------------------------------------------------
+
+def test_jw_known_answers(qf):
+    a = qf.SQOperator()
+    a.add(1.00, [2, 3], [])
+    a.add(1.00, [1], [3, 4])
+    aop = a.jw_transform()
+    expect = qf.QubitOperator()
+    for c, s in zip([-0.25j, -0.25, 0.25, -0.25j, -0.125j, 0.125, -0.125, -0.125j, -0.125, -0.125j, 0.125j, -0.125],
+                    ["Y_2 X_3", "Y_2 Y_3", "X_2 X_3", "X_2 Y_3", "Z_0 X_1 Y_3 X_4", "Z_0 X_1 Y_3 Y_4",
+                     "Z_0 X_1 X_3 X_4", "Z_0 X_1 X_3 Y_4", "Z_0 Y_1 Y_3 X_4", "Z_0 Y_1 Y_3 Y_4",
+                     "Z_0 Y_1 X_3 X_4", "Z_0 Y_1 X_3 Y_4"]):
+        expect.add(c, qf.build_circuit(s))
+    assert aop.check_op_equivalence(expect, True) is True
+
+
+@pytest.mark.parametrize("qubit_excitation", [False, True])
+def test_jw_matches_reference(qf, ref, qubit_excitation):
+    r = np.random.default_rng(17)
+    for trial in range(25):
+        sa, sb = qf.SQOperator(), ref.SQOperator()
+        for _ in range(int(r.integers(1, 4))):
+            nc, na = int(r.integers(0, 3)), int(r.integers(0, 3))
+            cre = [int(v) for v in r.integers(0, 7, size=nc)]
+            ann = [int(v) for v in r.integers(0, 7, size=na)]
+            c = complex(r.normal(), r.normal())
+            sa.add(c, cre, ann)
+            sb.add(c, cre, ann)
+        qa, qb = sa.jw_transform(qubit_excitation), sb.jw_transform(qubit_excitation)
+        assert sa.str() == sb.str()
+        assert terms_of(qa) == terms_of(qb)
+
+
+# ---- pools (sq_op_pool.cc, qubit_op_pool.cc) ------------------------------------------------
+
+@pytest.mark.parametrize("pool_type", ["S", "SD", "SDT", "SDTQ", "GSD", "sa_SD"])
+@pytest.mark.parametrize("occ", [[1, 1, 0, 0, 0, 0], [1, 1, 1, 1, 0, 0, 0, 0], [1, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0],
+                                 [1, 1, 1, 0, 0, 0, 0, 0], [1, 0, 0, 1, 0, 0, 1, 1]])
+def test_pools_match_reference(qf, ref, pool_type, occ):
+    pa, pb = qf.SQOpPool(), ref.SQOpPool()
+    pa.set_orb_spaces(occ)
+    pb.set_orb_spaces(occ)
+    if pool_type == "sa_SD" and sum(occ[0::2]) != sum(occ[1::2]):
+        with pytest.raises(ValueError):
+            pa.fill_pool(pool_type)
+        return
+    pa.fill_pool(pool_type)
+    pb.fill_pool(pool_type)
+    assert len(pa) == len(pb)
+    assert pa.str() == pb.str()
+    for order in ("commuting_grp_lex", "unique_lex"):
+        assert terms_of(pa.get_qubit_operator(order)) == terms_of(pb.get_qubit_operator(order))
+
+
+def test_pool_symmetry_filter_matches_reference(qf, ref):
+    occ = [1, 1, 1, 1, 0, 0, 0, 0, 0, 0]
+    irreps = [0, 1, 0, 1, 1]
+    for pool_type in ("SD", "GSD", "sa_SD"):
+        pa, pb = qf.SQOpPool(), ref.SQOpPool()
+        pa.set_orb_spaces(occ, irreps)
+        pb.set_orb_spaces(occ, irreps)
+        pa.fill_pool(pool_type)
+        pb.fill_pool(pool_type)
+        assert pa.str() == pb.str()
+
+
+def test_qubit_op_pools_match_reference(qf, ref):
+    for ptype in ("complete_qubit", "cqoy"):
+        a, b = qf.QubitOpPool(), ref.QubitOpPool()
+        a.fill_pool(ptype, 3)
+        b.fill_pool(ptype, 3)
+        assert a.str() == b.str()
+    h = W.to_operator(qf, W.random_pauli_sum(4, 6, seed=3, wmax=4))
+    hr = W.to_operator(ref, W.random_pauli_sum(4, 6, seed=3, wmax=4))
+    sqa, sqb = qf.SQOpPool(), ref.SQOpPool()
+    sqa.set_orb_spaces([1, 1, 0, 0])
+    sqb.set_orb_spaces([1, 1, 0, 0])
+    sqa.fill_pool("SD")
+    sqb.fill_pool("SD")
+    ca, cb = sqa.get_qubit_op_pool(), sqb.get_qubit_op_pool()
+    ca.join_as_commutator(h)
+    cb.join_as_commutator(hr)
+    assert ca.str() == cb.str()
+
+
+def test_hamiltonian_from_json_matches_reference(qf, ref):
+    # create_external_mol restated; JW of the He/cc-pVDZ integrals
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "He-ccpvdz.json")
+    mol = qf.system_factory(build_type="external", filename=path)
+    data = json.load(open(path))
+    H = ref.SQOperator()
+    H.add(data["scalar_energy"]["data"], [], [])
+    for p, q, h in data["oei"]["data"]:
+        H.add(h, [p], [q])
+    for p, q, r, s, h in data["tei"]["data"]:
+        H.add(h / 4.0, [p, q], [s, r])
+    assert terms_of(mol.hamiltonian) == terms_of(H.jw_transform())
+    assert mol.hf_reference == [1, 1, 0, 0, 0, 0, 0, 0, 0, 0]
+
+
+def test_exponentiate_gate_list_matches_construction(qf, ref, oracle):
+    # the gate list of exponentiate_pauli_string (exponentiate.py:34-85)
+    for word in ["X_0", "Y_1 Z_3", "Z_0 X_2 Y_4 Z_5", "Y_3 X_1"]:
+        term = qf.build_circuit(word)
+        circ, ph = qf.exponentiate_pauli_string(-0.37j, term)
+        letters = [(tok.split("_")[0], int(tok.split("_")[1])) for tok in word.split()]
+        expect = oracle.exponentiate_gates(-0.37j, letters)
+        assert [g.gate_id() for g in circ.gates()] == [g.gtype for g in expect]
+        assert [(g.target(), g.control()) for g in circ.gates()] == [(g.target, g.control) for g in expect]
+        assert circ.num_exponential_segments() == 1 and ph == 1.0
