@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""Benchmark of the Computer hot path on one MI355X.
+
+Workload (one "step"): a 30-qubit complex128 state (16 GiB) reset to |0...0>,
+then 40 brickwork layers (per-qubit Ry(U[0,2pi)) Rz(U[0,2pi)), then cX on
+alternating neighbour pairs, control = lower index), then exp(-i theta P) for
+2000 random Pauli strings of weight 4-8 (theta ~ N(0, 0.1)), then <psi|H|psi>
+for a random 1000-term Pauli sum (weight 1-6, N(0,1) coefficients).  Inputs
+are resident in HBM when the timed region starts.
+
+``value`` is the achieved HBM bandwidth of the whole step: the algorithmic
+bytes of every pass the engine launched (2 * 16 * 2^n per read-write sweep,
+16 * 2^n per read-only expectation pass over one X-mask group) divided by the
+step time.  ``roofline`` reports the dominant kernel (the single Pauli
+exponential sweep) timed with HIP events on the engine stream, against the
+8 TB/s HBM peak.  ``cpu_baseline`` times the reference's own C++ Computer
+(compiled from its sources into oracle/_ref) single-threaded on a bounded
+26-qubit sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
+
+
+def build_workload(qf, W, n, nlayers, nexp, nterms):
+    layers = W.brickwork_layers(n, nlayers, seed=101)
+    strings = W.random_pauli_strings(n, nexp, seed=102)
+    terms = W.random_pauli_sum(n, nterms, seed=103)
+    layer_circ = W.to_circuit(qf, layers)
+    U, phase = qf.trotterize(W.exp_operator(qf, strings))
+    assert phase == 1.0
+    H = W.to_operator(qf, terms)
+    return layers, strings, terms, layer_circ, U, H
+
+
+def time_dominant_kernel(qf, W, qc, n, reps=40):
+    """Average duration of the single Pauli-exponential sweep (one op per
+    pass) over `reps` back-to-back launches, HIP events on the engine stream."""
+    strings = W.random_pauli_strings(n, reps, seed=104)
+    U, _ = qf.trotterize(W.exp_operator(qf, strings))
+    saved = qf.get_tuning()
+    qf.set_tuning(0, 0, 1)  # one op per pass: every exponential is its own sweep
+    try:
+        qc.apply_circuit(U)  # warm
+        qf.synchronize()
+        t = qf.DeviceTimer()
+        t.start()
+        qc.apply_circuit(U)
+        ms = t.stop()
+    finally:
+        qf.set_tuning(*saved)
+    return ms / reps
+
+
+def cpu_baseline(W, n_cpu, oracle_ref_loader):
+    """Reference Computer (oracle/_ref), one thread, on a bounded sample:
+    1 brickwork layer + 5 exponentials + 5 expectation terms at n_cpu qubits."""
+    ref = oracle_ref_loader()
+    if ref is None:
+        return None
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("sv_oracle", os.path.join(ROOT, "oracle", "sv_oracle.py"))
+    oracle = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(oracle)
+    layers = W.brickwork_layers(n_cpu, 1, seed=101)
+    strings = W.random_pauli_strings(n_cpu, 5, seed=102)
+    terms = W.random_pauli_sum(n_cpu, 5, seed=103)
+    circ = W.to_circuit(ref, layers)
+    exp_circ = ref.Circuit()
+    for theta, term in strings:
+        for g in oracle.exponentiate_gates(-1j * theta, term):
+            exp_circ.add(ref.gate(g.gtype, g.target, g.control, g.parameter) if g.gtype in ("Rx", "Ry", "Rz")
+                         else ref.gate(g.gtype, g.target, g.control))
+    H = W.to_operator(ref, terms)
+    qc = ref.Computer(n_cpu)
+    t0 = time.perf_counter()
+    qc.apply_circuit(circ)
+    qc.apply_circuit(exp_circ)
+    e = qc.direct_op_exp_val(H)
+    dt = time.perf_counter() - t0
+    S = 16.0 * (1 << n_cpu)
+    # same accounting as the GPU's effective rate: 2S per gate or exponential, S per term
+    eff_bytes = 2 * S * (len(layers) + len(strings)) + S * len(terms)
+    return {
+        "seconds": dt,
+        "eff_bytes": eff_bytes,
+        "gates": len(layers),
+        "exps": len(strings),
+        "terms": len(terms),
+        "energy": complex(e),
+        "reference_gate_sweeps": len(layers) + exp_circ.size() + sum(len(t[1]) + 4 for t in terms),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nqubit", type=int, default=30)
+    ap.add_argument("--layers", type=int, default=40)
+    ap.add_argument("--nexp", type=int, default=2000)
+    ap.add_argument("--nterms", type=int, default=1000)
+    ap.add_argument("--cpu-nqubit", type=int, default=26)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the dominant-kernel timing")
+    ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the state-vector path is single-GPU; see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import qforte_amd as qf
+    from qforte_amd.engine import workloads as W
+
+    if qf.device_count() < 1:
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    if args.tuning:
+        qf.set_tuning(*[int(v) for v in args.tuning.split(",")])
+
+    n = args.nqubit
+    print(f"[bench] building workload n={n} layers={args.layers} exps={args.nexp} terms={args.nterms}",
+          file=sys.stderr, flush=True)
+    layers, strings, terms, layer_circ, U, H = build_workload(qf, W, n, args.layers, args.nexp, args.nterms)
+    qc = qf.Computer(n)
+
+    def step():
+        qc.reset()
+        qc.apply_circuit(layer_circ)
+        qc.apply_circuit(U)
+        return qc.direct_op_exp_val(H)
+
+    for i in range(args.warmup):
+        step()
+        qf.synchronize()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    qf.reset_counters()
+    qf.synchronize()
+    t0 = time.perf_counter()
+    energy = None
+    for i in range(args.steps):
+        energy = step()
+        qf.synchronize()
+        print(f"[bench] step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    counters = qf.get_counters()
+    norm_err = abs(qf.norm2(qc) - 1.0)
+    S = 16.0 * (1 << n)
+    ms_per_step = 1000.0 * dt / args.steps
+    gbps = counters["bytes"] / dt / 1e9
+    eff_bytes = 2 * S * (len(layers) + len(strings)) + S * len(terms)
+    eff_gbps = eff_bytes * args.steps / dt / 1e9
+
+    roofline = None
+    if not args.no_roofline:
+        avg_ms = time_dominant_kernel(qf, W, qc, n)
+        achieved = 2 * S / (avg_ms * 1e-3) / 1e9
+        roofline = {
+            "kernel": "k_pexp (single Pauli-exponential sweep, 2S = %.2f GB per launch)" % (2 * S / 1e9),
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4),
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from conftest import load_reference
+
+        print(f"[bench] cpu baseline: reference Computer at {args.cpu_nqubit} qubits", file=sys.stderr, flush=True)
+        res = cpu_baseline(W, args.cpu_nqubit, load_reference)
+        if res is not None:
+            cpu = {
+                "value": round(res["eff_bytes"] / res["seconds"] / 1e9, 3),
+                "unit": "GB/s effective (2S per gate or exponential + S per expectation term)",
+                "cores": 1,
+                "kind": "reference",
+                "sample": (f"reference C++ Computer (oracle/_ref, default non-OpenMP build) at {args.cpu_nqubit} "
+                           f"qubits: 1 brickwork layer ({res['gates']} gates) + {res['exps']} Pauli exponentials "
+                           f"(gate ladders) + {res['terms']}-term expectation, {res['seconds']:.2f} s"),
+                "seconds": round(res["seconds"], 3),
+                "gpu_effective_GBps_same_unit": round(eff_gbps, 1),
+            }
+    line = {
+        "metric": METRIC,
+        "value": round(gbps, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128 (f64)",
+        "data": "synthetic (seeded brickwork layers, random Pauli strings and Pauli sum)",
+        "config": {
+            "workload": f"{n}-qubit roofline: {args.layers} brickwork layers + {args.nexp} Pauli exponentials "
+                        f"(weight 4-8) + {args.nterms}-term expectation",
+            "nqubit": n,
+            "state_bytes": int(S),
+            "tuning": list(qf.get_tuning()),
+        },
+        "pct_of_peak": round(100.0 * gbps / HBM_PEAK_GBPS, 2),
+        "effective_GBps": round(eff_gbps, 1),
+        "passes_per_step": {
+            "fused": counters["fused_passes"] // args.steps,
+            "single": counters["single_passes"] // args.steps,
+            "expectation_groups": counters["reduce_passes"] // args.steps,
+        },
+        "energy": [energy.real, energy.imag],
+        "norm_error": norm_err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

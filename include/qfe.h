@@ -86,6 +86,12 @@ int qfe_synchronize(void);
  * counterpart (autotuner hook). */
 int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass);
 int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass);
+/* HIP events on the library stream, for timing kernels from the host
+ * (no reference counterpart). */
+int qfe_event_create(void** event);
+int qfe_event_record(void* event);
+int qfe_event_elapsed_ms(void* start, void* stop, float* ms);
+int qfe_event_destroy(void* event);
 /* Counters of launched passes since the last reset (fused passes, single-op
  * passes, expectation passes) and the algorithmic bytes they moved. */
 int qfe_get_counters(uint64_t* fused_passes, uint64_t* single_passes, uint64_t* reduce_passes, double* bytes);

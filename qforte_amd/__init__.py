@@ -18,6 +18,7 @@ from ._core import (  # noqa: F401
     BatchEngine,
     Circuit,
     Computer,
+    DeviceTimer,
     Gate,
     QubitBasis,
     QubitOperator,
@@ -33,6 +34,7 @@ from ._core import (  # noqa: F401
     get_counters,
     get_tuning,
     inner_product,
+    norm2,
     normalize_angle,
     pauli_masks,
     prepare_computer_from_circuit,

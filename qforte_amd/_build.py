@@ -6,8 +6,9 @@
   pybind11 bindings, compiled with ``g++`` and linked against ``libqfe.so``
   through an ``$ORIGIN`` rpath.
 
-Run ``python -m qforte_amd._build`` (or ``__graft_entry__.build()``).  Every
-call recompiles.
+Run ``python qforte_amd/_build.py`` (or ``__graft_entry__.build()``); it does
+not import the package, which needs the libraries it builds.  Every call
+recompiles.
 """
 
 from __future__ import annotations

@@ -159,6 +159,31 @@ class BatchEngine {
     std::vector<std::shared_ptr<UCCProgram>> keep_;
 };
 
+// HIP-event timer on the engine stream.
+class DeviceTimer {
+  public:
+    DeviceTimer() {
+        check(qfe_event_create(&a_), "DeviceTimer");
+        check(qfe_event_create(&b_), "DeviceTimer");
+    }
+    ~DeviceTimer() {
+        qfe_event_destroy(a_);
+        qfe_event_destroy(b_);
+    }
+    void start() { check(qfe_event_record(a_), "DeviceTimer.start"); }
+    float stop() {
+        check(qfe_event_record(b_), "DeviceTimer.stop");
+        float ms = 0.0f;
+        py::gil_scoped_release rel;
+        check(qfe_event_elapsed_ms(a_, b_, &ms), "DeviceTimer.stop");
+        return ms;
+    }
+
+  private:
+    void* a_ = nullptr;
+    void* b_ = nullptr;
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_core, m) {
@@ -381,6 +406,11 @@ PYBIND11_MODULE(_core, m) {
         return comp;
     }, "gate"_a, "computer"_a);
     m.def("inner_product", &qfa::dot, "a"_a, "b"_a);
+    m.def("norm2", [](const Computer& c) {
+        double v = 0.0;
+        check(qfe_state_norm2(c.handle(), &v), "norm2");
+        return v;
+    }, "Squared norm of a Computer's state, reduced on the device");
     m.def("find_irrep", &find_irrep, py::arg("orb_irrep_to_int"), py::arg("spinorb_indices"));
     m.def("normalize_angle", &normalize_angle);
 
@@ -439,6 +469,10 @@ PYBIND11_MODULE(_core, m) {
     });
     m.def("reset_counters", []() { qfe_reset_counters(); });
     m.def("synchronize", []() { check(qfe_synchronize(), "synchronize"); });
+    py::class_<DeviceTimer>(m, "DeviceTimer")
+        .def(py::init<>())
+        .def("start", &DeviceTimer::start)
+        .def("stop", &DeviceTimer::stop, "Milliseconds between start and stop on the engine stream");
     m.def("stream_handle", []() {
         void* s = nullptr;
         check(qfe_get_stream(&s), "stream_handle");

@@ -293,6 +293,35 @@ int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass) {
     return QFE_OK;
 }
 
+int qfe_event_create(void** event) {
+    if (!event) return fail(QFE_ERR_ARG, "qfe_event_create: null pointer");
+    int rc = ensure_init();
+    if (rc) return rc;
+    hipEvent_t e;
+    QFE_HIP(hipEventCreate(&e));
+    *event = (void*)e;
+    return QFE_OK;
+}
+
+int qfe_event_record(void* event) {
+    if (!event) return fail(QFE_ERR_ARG, "qfe_event_record: null event");
+    QFE_HIP(hipEventRecord((hipEvent_t)event, g_ctx.stream));
+    return QFE_OK;
+}
+
+int qfe_event_elapsed_ms(void* start, void* stop, float* ms) {
+    if (!start || !stop || !ms) return fail(QFE_ERR_ARG, "qfe_event_elapsed_ms: invalid argument");
+    QFE_HIP(hipEventSynchronize((hipEvent_t)stop));
+    QFE_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return QFE_OK;
+}
+
+int qfe_event_destroy(void* event) {
+    if (!event) return QFE_OK;
+    QFE_HIP(hipEventDestroy((hipEvent_t)event));
+    return QFE_OK;
+}
+
 int qfe_get_counters(uint64_t* fused_passes, uint64_t* single_passes, uint64_t* reduce_passes, double* bytes) {
     if (fused_passes) *fused_passes = g_ctx.counters.fused;
     if (single_passes) *single_passes = g_ctx.counters.single;
