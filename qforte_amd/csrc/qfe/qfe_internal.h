@@ -52,6 +52,10 @@ Context& ctx();
 int ensure_init();
 int scratch_reserve(size_t bytes, void** out);
 int pinned_reserve(size_t bytes, void** out);
+// Double-buffered pinned staging: fill *host (bytes), then commit copies it to
+// dev on the library stream without blocking the host.
+int staging_begin(size_t bytes, void** host);
+int staging_commit(void* dev, size_t bytes);
 
 }  // namespace qfe
 

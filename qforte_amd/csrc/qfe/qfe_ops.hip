@@ -336,30 +336,33 @@ struct PassDesc {
 };
 
 template <int KQ>
-__global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, const PassDesc* __restrict__ pd,
-                                                     const DevOp* __restrict__ ops) {
+__global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
+                                                     const DevOp* __restrict__ ops, uint32_t nops) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
     constexpr uint32_t T = 1u << KQ;
-    const uint32_t nlow = pd->nlow;
     const uint32_t nhi = KQ - nlow;
-    // tile base: scatter blockIdx into the non-tile bit positions
-    uint64_t base = 0;
-    {
-        const uint64_t g = blockIdx.x;
-        const uint32_t nr = pd->nrest;
-        for (uint32_t i = 0; i < nr; ++i) base |= ((g >> i) & 1ull) << pd->restpos[i];
+    const uint64_t himask = bmask & ~((1ull << nlow) - 1ull);
+    // tile base: blockIdx with a zero bit inserted at every tile qubit
+    // (the low nlow tile qubits are a plain shift); pure scalar ALU
+    uint64_t base = (uint64_t)blockIdx.x << nlow;
+    for (uint64_t m = himask; m; m &= m - 1) {
+        const uint32_t p = (uint32_t)__builtin_ctzll(m);
+        const uint64_t lo = base & ((1ull << p) - 1ull);
+        base = ((base ^ lo) << 1) | lo;
     }
-    // offsets of the high tile bits (2^nhi entries, nhi <= 9)
+    // offsets of the high tile qubits (2^nhi entries, nhi <= 9)
     for (uint32_t h = threadIdx.x; h < (1u << nhi); h += kFusedTPB) {
         uint64_t off = 0;
-        for (uint32_t i = 0; i < nhi; ++i) off |= ((uint64_t)((h >> i) & 1u)) << pd->hipos[i];
+        uint32_t i = 0;
+        for (uint64_t m = himask; m; m &= m - 1, ++i)
+            if ((h >> i) & 1u) off |= 1ull << __builtin_ctzll(m);
         hioff[h] = off;
     }
     __syncthreads();
     const uint32_t lowmask = (1u << nlow) - 1u;
+#pragma unroll
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) tile[l] = s[base | hioff[l >> nlow] | (l & lowmask)];
-    const uint32_t nops = pd->nops;
     for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
         const DevOp& o = ops[oi];
@@ -496,13 +499,175 @@ static DevOp localize(const DevOp& g, uint64_t bmask, const int* local_of) {
     return o;
 }
 
+// ---------------------------------------------------------------------------
+// Device: register-orbit pass for runs of Pauli exponentials
+//
+// The flip masks x_1..x_r of a run of exponentials span a space of dimension
+// M <= 4 over GF(2).  With a reduced basis b_0..b_{M-1} (pivot = highest bit,
+// each pivot bit present in one basis vector only), every index j with all
+// pivot bits clear owns the orbit {j ^ c_i}, c_i = XOR of b_k over the bits k
+// of i.  One thread loads its 2^M amplitudes into registers, applies every
+// exponential of the run there (exponential o pairs i with i ^ v_o, v_o its
+// mask in orbit coordinates) and stores them: one streaming sweep for the
+// whole run, no LDS and no barriers.
+// ---------------------------------------------------------------------------
+
+constexpr int kOrbitMax = 4;
+constexpr int kOrbitTPB = 256;
+
+struct OrbitOp {
+    uint32_t v;      // mask in orbit coordinates (0: diagonal exponential)
+    uint32_t smask;  // bit i = parity(c_i & z)
+    uint64_t z;
+    double c, sn;
+    double2 ph;
+};
+
+struct OrbitDesc {
+    uint64_t c[1 << kOrbitMax];  // orbit element offsets
+    uint32_t piv[kOrbitMax];     // pivot bits, ascending
+};
+
+template <int M, int V>
+__device__ __forceinline__ void orbit_pexp_v(double2 (&a)[1 << M], bool pj, uint32_t smask, double c, double sn,
+                                             double2 ph) {
+    constexpr int HB = 31 - __builtin_clz((unsigned)V);
+#pragma unroll
+    for (int i = 0; i < (1 << M); ++i) {
+        if (i & (1 << HB)) continue;
+        const int k = i ^ V;
+        const bool na = pj != (((smask >> i) & 1u) != 0);
+        const bool nb = pj != (((smask >> k) & 1u) != 0);
+        pexp_pair(a[i], a[k], c, sn, ph, na, nb);
+    }
+}
+
+template <int M, int V>
+__device__ __forceinline__ void orbit_pexp_dispatch(uint32_t v, double2 (&a)[1 << M], bool pj, uint32_t smask,
+                                                    double c, double sn, double2 ph) {
+    if constexpr (V < (1 << M)) {
+        if (v == (uint32_t)V) {
+            orbit_pexp_v<M, V>(a, pj, smask, c, sn, ph);
+            return;
+        }
+        orbit_pexp_dispatch<M, V + 1>(v, a, pj, smask, c, sn, ph);
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, uint64_t norbit, OrbitDesc d,
+                                                     const OrbitOp* __restrict__ ops, uint32_t nops) {
+    const uint64_t t = (uint64_t)blockIdx.x * kOrbitTPB + threadIdx.x;
+    if (t >= norbit) return;
+    uint64_t j = t;
+#pragma unroll
+    for (int k = 0; k < M; ++k) j = ins0(j, d.piv[k]);
+    double2 a[1 << M];
+#pragma unroll
+    for (int i = 0; i < (1 << M); ++i) a[i] = s[j ^ d.c[i]];
+    for (uint32_t o = 0; o < nops; ++o) {
+        const OrbitOp op = ops[o];
+        const bool pj = __popcll(j & op.z) & 1;
+        if (op.v == 0) {
+            const double2 fp = make_double2(op.c - op.sn * op.ph.y, op.sn * op.ph.x);
+            const double2 fm = make_double2(op.c + op.sn * op.ph.y, -op.sn * op.ph.x);
+#pragma unroll
+            for (int i = 0; i < (1 << M); ++i) {
+                const bool neg = pj != (((op.smask >> i) & 1u) != 0);
+                a[i] = cmul(neg ? fm : fp, a[i]);
+            }
+        } else {
+            orbit_pexp_dispatch<M, 1>(op.v, a, pj, op.smask, op.c, op.sn, op.ph);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < (1 << M); ++i) s[j ^ d.c[i]] = a[i];
+}
+
+// Reduced GF(2) basis with pivot = highest bit; every pivot bit appears in
+// exactly one basis vector.
+struct Span {
+    std::vector<uint64_t> b;
+    uint64_t reduce(uint64_t x) const {
+        for (uint64_t v : b)
+            if ((x >> (63 - __builtin_clzll(v))) & 1ull) x ^= v;
+        return x;
+    }
+    bool add(uint64_t x, int maxdim) {
+        const uint64_t r = reduce(x);
+        if (!r) return true;
+        if ((int)b.size() >= maxdim) return false;
+        const int p = 63 - __builtin_clzll(r);
+        for (uint64_t& v : b)
+            if ((v >> p) & 1ull) v ^= r;
+        b.push_back(r);
+        return true;
+    }
+};
+
+enum PassKind { P_SINGLE = 0, P_TILE = 1, P_ORBIT = 2 };
+
 struct Pass {
-    uint64_t bmask;
+    int kind = P_SINGLE;
+    uint64_t bmask = 0;
     std::vector<int> ops;  // indices into the lowered op list
+    Span span;             // P_ORBIT
 };
 
 // Greedy planner.  Ops may move ahead of an earlier op only when their qubit
-// supports are disjoint (then the two commute exactly).
+// supports are disjoint (then the two commute exactly).  Each pass is grown
+// two ways from the first pending op -- as an LDS tile (mixing qubits fit
+// 2^k amplitudes) and as a register orbit (a run of exponentials whose flip
+// masks span <= 4 dimensions) -- and the one covering more ops is kept.
+static void grow_tile(const std::vector<HostOp>& ops, const std::vector<char>& done, size_t first, int K,
+                      uint64_t lowmask, uint64_t all, int max_ops, Pass& p) {
+    p = Pass{};
+    p.kind = P_TILE;
+    p.bmask = lowmask;
+    uint64_t blocked = 0;
+    for (size_t i = first; i < ops.size(); ++i) {
+        if (done[i]) continue;
+        const HostOp& h = ops[i];
+        if (h.support & blocked) {
+            blocked |= h.support;
+            if ((blocked & all) == all) break;
+            continue;
+        }
+        const uint64_t nb = p.bmask | h.mix;
+        if (__builtin_popcountll(nb) <= K && (int)p.ops.size() < max_ops) {
+            p.bmask = nb;
+            p.ops.push_back((int)i);
+        } else {
+            blocked |= h.support;
+            if ((blocked & all) == all || (int)p.ops.size() >= max_ops) break;
+        }
+    }
+}
+
+static void grow_orbit(const std::vector<HostOp>& ops, const std::vector<char>& done, size_t first, uint64_t all,
+                       int max_ops, Pass& p) {
+    p = Pass{};
+    p.kind = P_ORBIT;
+    uint64_t blocked = 0;
+    for (size_t i = first; i < ops.size(); ++i) {
+        if (done[i]) continue;
+        const HostOp& h = ops[i];
+        if (h.support & blocked) {
+            blocked |= h.support;
+            if ((blocked & all) == all) break;
+            continue;
+        }
+        Span trial = p.span;
+        if (h.op.kind == K_PEXP && (int)p.ops.size() < max_ops && trial.add(h.op.x, kOrbitMax)) {
+            p.span = trial;
+            p.ops.push_back((int)i);
+        } else {
+            blocked |= h.support;
+            if ((blocked & all) == all || (int)p.ops.size() >= max_ops) break;
+        }
+    }
+}
+
 static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::vector<Pass>& passes) {
     const int K = std::min(tu.tile_qubits, n);
     const int L = std::max(std::min(tu.low_qubits, K), K - 9);
@@ -513,40 +678,82 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
     while (first < ops.size()) {
         while (first < ops.size() && done[first]) ++first;
         if (first >= ops.size()) break;
+        Pass tile, orbit;
+        grow_tile(ops, done, first, K, lowmask, all, tu.max_ops, tile);
+        grow_orbit(ops, done, first, all, tu.max_ops, orbit);
         Pass p;
-        p.bmask = lowmask;
-        uint64_t blocked = 0;
-        for (size_t i = first; i < ops.size(); ++i) {
-            if (done[i]) continue;
-            const HostOp& h = ops[i];
-            if (h.support & blocked) {
-                blocked |= h.support;
-                if ((blocked & all) == all) break;
-                continue;
-            }
-            const uint64_t nb = p.bmask | h.mix;
-            if (__builtin_popcountll(nb) <= K && (int)p.ops.size() < tu.max_ops) {
-                p.bmask = nb;
-                p.ops.push_back((int)i);
-                done[i] = 1;
-            } else if (p.ops.empty()) {
-                // does not fit a tile even alone: a standalone streaming pass
-                p.ops.push_back((int)i);
-                done[i] = 1;
-                p.bmask = 0;
-                break;
-            } else {
-                blocked |= h.support;
-                if ((blocked & all) == all) break;
-                if ((int)p.ops.size() >= tu.max_ops) break;
-            }
+        if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
+            p = std::move(orbit);
+        } else if (tile.ops.size() >= 2) {
+            p = std::move(tile);
+        } else {
+            // a standalone streaming pass
+            p = Pass{};
+            p.kind = P_SINGLE;
+            p.ops.push_back((int)first);
         }
+        for (int i : p.ops) done[i] = 1;
         passes.push_back(std::move(p));
     }
 }
 
+template <int M>
+static void launch_orbit_k(double2* s, uint64_t norbit, const OrbitDesc& d, const OrbitOp* ops, uint32_t nops,
+                           hipStream_t st) {
+    const unsigned blocks = (unsigned)((norbit + kOrbitTPB - 1) / kOrbitTPB);
+    hipLaunchKernelGGL(k_orbit<M>, dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, d, ops, nops);
+}
+
+static int launch_orbit(qfe_state_s* st, int M, const OrbitDesc& d, const OrbitOp* ops, uint32_t nops) {
+    Context& cx = ctx();
+    const uint64_t norbit = st->N >> M;
+    switch (M) {
+        case 1: launch_orbit_k<1>(st->d, norbit, d, ops, nops, cx.stream); break;
+        case 2: launch_orbit_k<2>(st->d, norbit, d, ops, nops, cx.stream); break;
+        case 3: launch_orbit_k<3>(st->d, norbit, d, ops, nops, cx.stream); break;
+        case 4: launch_orbit_k<4>(st->d, norbit, d, ops, nops, cx.stream); break;
+        default: return fail(QFE_ERR_STATE, "internal: orbit dimension out of range");
+    }
+    QFE_HIP(hipGetLastError());
+    cx.counters.fused++;
+    cx.counters.bytes += 32.0 * (double)st->N;
+    return QFE_OK;
+}
+
+// Orbit descriptor and per-op orbit data for a finished orbit pass.
+static void build_orbit(const Pass& p, const std::vector<HostOp>& ops, OrbitDesc& d, std::vector<OrbitOp>& out) {
+    std::vector<uint64_t> b = p.span.b;
+    std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
+    const int M = (int)b.size();
+    d = OrbitDesc{};
+    for (int k = 0; k < M; ++k) d.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+    for (int i = 0; i < (1 << M); ++i) {
+        uint64_t c = 0;
+        for (int k = 0; k < M; ++k)
+            if ((i >> k) & 1) c ^= b[k];
+        d.c[i] = c;
+    }
+    for (int oi : p.ops) {
+        const DevOp& g = ops[oi].op;
+        OrbitOp o{};
+        uint32_t v = 0;
+        for (int k = 0; k < M; ++k)
+            if ((g.x >> d.piv[k]) & 1ull) v |= 1u << k;
+        o.v = v;
+        uint32_t sm = 0;
+        for (int i = 0; i < (1 << M); ++i)
+            if (__builtin_popcountll(d.c[i] & g.z) & 1) sm |= 1u << i;
+        o.smask = sm;
+        o.z = g.z;
+        o.c = g.m[0].x;
+        o.sn = g.m[0].y;
+        o.ph = g.m[1];
+        out.push_back(o);
+    }
+}
+
 template <int KQ>
-static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc* pd, const DevOp* ops, hipStream_t st) {
+static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, hipStream_t st) {
     const size_t lds = (size_t)sizeof(double2) << KQ;
     static bool attr_set = false;  // large tiles need the dynamic-LDS limit raised once
     if (!attr_set) {
@@ -554,25 +761,27 @@ static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc* pd, con
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd, ops);
+    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops,
+                       pd.nops);
 }
 
-static int launch_fused(qfe_state_s* st, int k, const PassDesc* dpd, const DevOp* dops) {
+static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) {
     Context& cx = ctx();
+    const int k = (int)pd.k;
     const uint64_t nblocks = st->N >> k;
     switch (k) {
-        case 2: launch_fused_k<2>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 3: launch_fused_k<3>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 4: launch_fused_k<4>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 5: launch_fused_k<5>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 6: launch_fused_k<6>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 7: launch_fused_k<7>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 8: launch_fused_k<8>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 9: launch_fused_k<9>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 10: launch_fused_k<10>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 11: launch_fused_k<11>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 12: launch_fused_k<12>(st->d, nblocks, dpd, dops, cx.stream); break;
-        case 13: launch_fused_k<13>(st->d, nblocks, dpd, dops, cx.stream); break;
+        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, cx.stream); break;
         default: return fail(QFE_ERR_ARG, "internal: fused tile size out of range");
     }
     QFE_HIP(hipGetLastError());
@@ -589,20 +798,27 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     plan(ops, n, cx.tuning, passes);
 
     // Build all device descriptors for this call in one host buffer, upload once.
-    // Layout: [PassDesc per fused pass][DevOp arrays][U2 matrices for single passes]
+    // Layout: [PassDesc per tile pass][DevOp arrays][U2 matrices for single passes][OrbitOp arrays]
     std::vector<PassDesc> pdescs;
     std::vector<DevOp> dops;
     std::vector<size_t> pass_pd(passes.size(), (size_t)-1), pass_op0(passes.size(), 0);
     std::vector<size_t> single_mat(passes.size(), (size_t)-1);
     std::vector<double2> mats;
+    std::vector<OrbitDesc> odescs(passes.size());
+    std::vector<OrbitOp> oops;
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const Pass& p = passes[pi];
-        if (p.ops.size() == 1) {
+        if (p.kind == P_SINGLE) {
             const HostOp& h = ops[p.ops[0]];
             if (h.op.kind == K_U2) {
                 single_mat[pi] = mats.size();
                 for (int i = 0; i < 16; ++i) mats.push_back(h.op.m[i]);
             }
+            continue;
+        }
+        if (p.kind == P_ORBIT) {
+            pass_op0[pi] = oops.size();
+            build_orbit(p, ops, odescs[pi], oops);
             continue;
         }
         // extend the tile with the lowest free qubits up to K bits
@@ -640,31 +856,38 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     const size_t off_ops = (bytes_pd + 255) & ~(size_t)255;
     const size_t bytes_ops = dops.size() * sizeof(DevOp);
     const size_t off_mat = (off_ops + bytes_ops + 255) & ~(size_t)255;
-    const size_t total = off_mat + mats.size() * sizeof(double2);
+    const size_t bytes_mat = mats.size() * sizeof(double2);
+    const size_t off_orb = (off_mat + bytes_mat + 255) & ~(size_t)255;
+    const size_t bytes_orb = oops.size() * sizeof(OrbitOp);
+    const size_t total = off_orb + bytes_orb;
     char* dev = nullptr;
-    if (total) {
+    if (bytes_ops || bytes_mat || bytes_orb) {
         void* scr = nullptr;
         int rc = scratch_reserve(total, &scr);
         if (rc) return rc;
         dev = static_cast<char*>(scr);
-        std::vector<char> host(total);
-        if (bytes_pd) std::memcpy(host.data(), pdescs.data(), bytes_pd);
-        if (bytes_ops) std::memcpy(host.data() + off_ops, dops.data(), bytes_ops);
-        if (!mats.empty()) std::memcpy(host.data() + off_mat, mats.data(), mats.size() * sizeof(double2));
-        QFE_HIP(hipMemcpyAsync(dev, host.data(), total, hipMemcpyHostToDevice, cx.stream));
-        // the host staging buffer dies at scope exit: make sure the copy consumed it
-        QFE_HIP(hipStreamSynchronize(cx.stream));
+        char* host = nullptr;
+        rc = staging_begin(total, reinterpret_cast<void**>(&host));
+        if (rc) return rc;
+        if (bytes_pd) std::memcpy(host, pdescs.data(), bytes_pd);
+        if (bytes_ops) std::memcpy(host + off_ops, dops.data(), bytes_ops);
+        if (bytes_mat) std::memcpy(host + off_mat, mats.data(), bytes_mat);
+        if (bytes_orb) std::memcpy(host + off_orb, oops.data(), bytes_orb);
+        rc = staging_commit(dev, total);
+        if (rc) return rc;
     }
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const Pass& p = passes[pi];
         int rc;
-        if (p.ops.size() == 1) {
+        if (p.kind == P_SINGLE) {
             double2* m = single_mat[pi] == (size_t)-1 ? nullptr : reinterpret_cast<double2*>(dev + off_mat) + single_mat[pi];
             rc = launch_single(st, ops[p.ops[0]], m);
+        } else if (p.kind == P_ORBIT) {
+            const OrbitOp* oo = reinterpret_cast<const OrbitOp*>(dev + off_orb) + pass_op0[pi];
+            rc = launch_orbit(st, (int)p.span.b.size(), odescs[pi], oo, (uint32_t)p.ops.size());
         } else {
-            const PassDesc* dpd = reinterpret_cast<const PassDesc*>(dev) + pass_pd[pi];
             const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops) + pass_op0[pi];
-            rc = launch_fused(st, (int)pdescs[pass_pd[pi]].k, dpd, dop);
+            rc = launch_fused(st, pdescs[pass_pd[pi]], dop);
         }
         if (rc) return rc;
     }

@@ -75,6 +75,50 @@ int pinned_reserve(size_t bytes, void** out) {
     return QFE_OK;
 }
 
+// Pinned, double-buffered host staging for small host-to-device uploads: the
+// host fills one half while the copy out of the other may still be queued, so
+// planning the next call overlaps the kernels of the previous one.
+namespace {
+struct Staging {
+    void* buf[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    int cur = 0;
+};
+Staging g_staging;
+}  // namespace
+
+int staging_begin(size_t bytes, void** host) {
+    Staging& S = g_staging;
+    const int h = S.cur;
+    if (S.pending[h]) {
+        QFE_HIP(hipEventSynchronize(S.ev[h]));
+        S.pending[h] = false;
+    }
+    if (S.cap[h] < bytes) {
+        if (S.buf[h]) QFE_HIP(hipHostFree(S.buf[h]));
+        S.buf[h] = nullptr;
+        S.cap[h] = 0;
+        const size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
+        QFE_HIP(hipHostMalloc(&S.buf[h], want, hipHostMallocDefault));
+        S.cap[h] = want;
+    }
+    if (!S.ev[h]) QFE_HIP(hipEventCreateWithFlags(&S.ev[h], hipEventDisableTiming));
+    *host = S.buf[h];
+    return QFE_OK;
+}
+
+int staging_commit(void* dev, size_t bytes) {
+    Staging& S = g_staging;
+    const int h = S.cur;
+    QFE_HIP(hipMemcpyAsync(dev, S.buf[h], bytes, hipMemcpyHostToDevice, g_ctx.stream));
+    QFE_HIP(hipEventRecord(S.ev[h], g_ctx.stream));
+    S.pending[h] = true;
+    S.cur ^= 1;
+    return QFE_OK;
+}
+
 // ---- elementwise kernels ---------------------------------------------------------
 
 __global__ void k_fill_zero(double2* __restrict__ d, uint64_t N) {

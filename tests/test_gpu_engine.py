@@ -171,6 +171,50 @@ def test_trotterized_exponentials(qf, oracle, n, tuning):
     assert np.max(np.abs(qc.get_coeff_vec() - expect)) < AMP_TOL
 
 
+@pytest.mark.parametrize("n", [5, 13])
+def test_exponential_runs_with_dependent_masks(qf, oracle, n):
+    # runs whose flip masks repeat, combine (x3 = x1 ^ x2), vanish (pure Z
+    # strings) or interleave with gates exercise every orbit-coordinate case
+    r = np.random.default_rng(77)
+    seq = []
+    for _ in range(60):
+        kind = int(r.integers(6))
+        if kind == 0 and seq:
+            seq.append(("exp", float(r.normal()), seq[-1][2] if seq[-1][0] == "exp" else [("X", 0)]))
+        elif kind == 1:
+            qs = sorted(int(v) for v in r.choice(n, size=2, replace=False))
+            seq.append(("exp", float(r.normal()), [("Z", qs[0]), ("Z", qs[1])]))
+        elif kind == 2:
+            q = int(r.integers(n))
+            seq.append(("gate", ("H", q, q, 0.0)))
+        else:
+            w = int(r.integers(1, min(4, n) + 1))
+            qs = sorted(int(v) for v in r.choice(min(n, 6), size=min(w, min(n, 6)), replace=False))
+            seq.append(("exp", float(r.normal()), [(("X", "Y", "Z")[int(r.integers(3))], q) for q in qs]))
+    circ = qf.Circuit()
+    psi = rand_state(n, 9)
+    expect = psi.copy()
+    for item in seq:
+        if item[0] == "gate":
+            g = item[1]
+            circ.add(qf.gate(g[0], g[1], g[2]))
+            expect = oracle.apply_gate(expect, n, oracle.G(*g))
+        else:
+            _, theta, term = item
+            c = qf.Circuit()
+            for l, q in term:
+                c.add(qf.gate(l, q, q))
+            U, _ = qf.exponentiate_pauli_string(-1j * theta, c)
+            circ.add(U)
+            expect = oracle.exp_pauli(expect, -1j * theta, term)
+    for tuning in TUNINGS:
+        qf.set_tuning(*tuning)
+        qc = qf.Computer(n)
+        qc.set_coeff_vec(psi)
+        qc.apply_circuit(circ)
+        assert np.max(np.abs(qc.get_coeff_vec() - expect)) < AMP_TOL, tuning
+
+
 def test_trotter_known_answer(qf):
     # tests/test_trotter.py:15-47
     op = qf.QubitOperator()

@@ -1,0 +1,183 @@
+"""Compiled UCC programs: energy against the oracle and against the
+reference's gate-level path through Computer, the adjoint gradient against
+central finite differences and against the reference's measure_gradient
+recursion, and ADAPT pool gradients (measure_gradient3)."""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+E_TOL = 1e-10
+
+
+def terms_list(op):
+    return [(c, [(g.gate_id(), g.target()) for g in circ.gates()]) for c, circ in op.terms()]
+
+
+def oracle_ucc_energy(oracle, n, ref_occ, gens, t, ham_terms):
+    exps = []
+    for k, g in enumerate(gens):
+        for c, term in terms_list(g):
+            exps.append((t[k] * c, term))
+    psi = oracle.ucc_state(n, ref_occ, exps)
+    return float(np.real(oracle.expval(psi, n, ham_terms))), psi
+
+
+@pytest.fixture(scope="module")
+def h4(qf):
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(mol.hf_reference)
+    pool.fill_pool("SD")
+    return mol, pool
+
+
+def test_energy_matches_oracle_and_gate_path(qf, oracle, h4):
+    mol, pool = h4
+    n = len(mol.hf_reference)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    gens, _, _ = qf.ucc_generators(pool, tops)
+    t = list(np.random.default_rng(1).normal(0.0, 0.05, len(tops)))
+    e = prog.energy(t)
+    e_or, _ = oracle_ucc_energy(oracle, n, mol.hf_reference, gens, t, terms_list(mol.hamiltonian))
+    assert abs(e - e_or) < E_TOL
+    # the reference's own path: SQOpPool -> get_qubit_operator -> trotterize -> Computer
+    temp = qf.SQOpPool()
+    for tk, top in zip(t, tops):
+        temp.add(tk, pool[top][1])
+    A = temp.get_qubit_operator("commuting_grp_lex")
+    U, phase = qf.trotterize(A)
+    assert phase == 1.0
+    qc = qf.Computer(n)
+    qc.apply_circuit(qf.build_refprep(mol.hf_reference))
+    qc.apply_circuit(U)
+    e_gate = qc.direct_op_exp_val(mol.hamiltonian).real
+    assert abs(e - e_gate) < E_TOL
+
+
+def test_adjoint_gradient_matches_finite_differences(qf, h4):
+    mol, pool = h4
+    n = len(mol.hf_reference)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    t = np.random.default_rng(2).normal(0.0, 0.05, len(tops))
+    e0, g = prog.gradient(list(t))
+    assert abs(e0 - prog.energy(list(t))) < 1e-12
+    h = 1e-5
+    for k in range(len(tops)):
+        tp, tm = t.copy(), t.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (prog.energy(list(tp)) - prog.energy(list(tm))) / (2 * h)
+        assert abs(fd - g[k]) < 1e-8, (k, fd, g[k])
+
+
+def test_gradient_matches_reference_recursion(qf, h4):
+    # uccvqeabc.py:122-251 restated on the Computer API
+    mol, pool = h4
+    n = len(mol.hf_reference)
+    tops = list(range(len(pool)))
+    t = list(np.random.default_rng(3).normal(0.0, 0.05, len(tops)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    _, g = prog.gradient(t)
+    temp = qf.SQOpPool()
+    for tk, top in zip(t, tops):
+        temp.add(tk, pool[top][1])
+    U, _ = qf.trotterize(temp.get_qubit_operator("commuting_grp_lex"))
+    psi = qf.Computer(n)
+    psi.apply_circuit(qf.build_refprep(mol.hf_reference))
+    psi.apply_circuit(U)
+    sig = qf.Computer(psi)
+    sig.apply_operator(mol.hamiltonian)
+    M = len(tops)
+    grads = np.zeros(M)
+    K_prev = pool[tops[M - 1]][1].jw_transform()
+    K_prev.mult_coeffs(pool[tops[M - 1]][0])
+    tmp = qf.Computer(psi)
+    tmp.apply_operator(K_prev)
+    grads[M - 1] = 2.0 * np.real(np.vdot(sig.get_coeff_vec(), tmp.get_coeff_vec()))
+    for mu in reversed(range(M - 1)):
+        K = pool[tops[mu]][1].jw_transform()
+        K.mult_coeffs(pool[tops[mu]][0])
+        Umu, _ = qf.trotterize(K_prev, factor=-t[mu + 1])
+        sig.apply_circuit(Umu)
+        psi.apply_circuit(Umu)
+        tmp = qf.Computer(psi)
+        tmp.apply_operator(K)
+        grads[mu] = 2.0 * np.real(np.vdot(sig.get_coeff_vec(), tmp.get_coeff_vec()))
+        K_prev = K
+    assert np.max(np.abs(grads - np.array(g))) < 1e-10
+
+
+def test_pool_gradients_match_reference_formula(qf, h4):
+    # measure_gradient3 (uccvqeabc.py:253-283) over the GSD pool, batched
+    mol, pool = h4
+    n = len(mol.hf_reference)
+    tops = list(range(len(pool)))
+    t = list(np.random.default_rng(4).normal(0.0, 0.05, len(tops)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    gsd = qf.SQOpPool()
+    gsd.set_orb_spaces(mol.hf_reference)
+    gsd.fill_pool("GSD")
+    ops = []
+    for coeff, op in gsd:
+        q = op.jw_transform()
+        q.mult_coeffs(coeff)
+        ops.append(q)
+    got = prog.pool_gradients(t, ops, 64)
+    psi = qf.Computer(n)
+    prog.prepare(t, psi)
+    sig = qf.Computer(psi)
+    sig.apply_operator(mol.hamiltonian)
+    sv = sig.get_coeff_vec()
+    for k, q in enumerate(ops[:40]):
+        tmp = qf.Computer(psi)
+        tmp.apply_operator(q)
+        want = 2.0 * np.real(np.vdot(sv, tmp.get_coeff_vec()))
+        assert abs(got[k] - want) < 1e-10
+
+
+def test_synthetic_uccsd_energy_against_dense_oracle(qf, oracle):
+    # 6 spatial orbitals, 3 alpha + 3 beta electrons: 12 qubits
+    mol = qf.synthetic_molecule(6, 3, 3, seed=5)
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(mol.hf_reference)
+    pool.fill_pool("SD")
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(12, mol.hf_reference, mol.hamiltonian, pool, tops)
+    gens, _, _ = qf.ucc_generators(pool, tops)
+    t = list(np.random.default_rng(6).normal(0.0, 0.05, len(tops)))
+    e = prog.energy(t)
+    e_or, psi = oracle_ucc_energy(oracle, 12, mol.hf_reference, gens, t, terms_list(mol.hamiltonian))
+    assert abs(e - e_or) < E_TOL
+    qc = qf.Computer(12)
+    prog.prepare(t, qc)
+    assert np.max(np.abs(qc.get_coeff_vec() - psi)) < 1e-12
+
+
+def test_sa_sd_generators_share_amplitudes(qf, oracle, h4):
+    mol, _ = h4
+    n = len(mol.hf_reference)
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(mol.hf_reference)
+    pool.fill_pool("sa_SD")
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops, pool_type="sa_SD")
+    gens, idx, _ = qf.ucc_generators(pool, tops, pool_type="sa_SD")
+    assert len(gens) > len(tops)
+    t = np.random.default_rng(8).normal(0.0, 0.05, len(tops))
+    e = prog.energy(list(t))
+    tt = [t[i] for i in idx]
+    e_or, _ = oracle_ucc_energy(oracle, n, mol.hf_reference, gens, tt, terms_list(mol.hamiltonian))
+    assert abs(e - e_or) < E_TOL
+    _, g = prog.gradient(list(t))
+    h = 1e-5
+    for k in range(len(tops)):
+        tp, tm = t.copy(), t.copy()
+        tp[k] += h
+        tm[k] -= h
+        assert abs((prog.energy(list(tp)) - prog.energy(list(tm))) / (2 * h) - g[k]) < 1e-8
