@@ -31,6 +31,10 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
     # keep packed FP32 instructions out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Keep LLVM from merging the per-mask branches of the register-orbit
+    # kernel into one block with a variable array index; that index forces the
+    # amplitude array into scratch memory.
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wno-unused-result",
 ]
 CXX = os.environ.get("CXX", "g++")

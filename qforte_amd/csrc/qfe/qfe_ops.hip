@@ -524,7 +524,8 @@ struct OrbitOp {
 };
 
 struct OrbitDesc {
-    uint64_t c[1 << kOrbitMax];  // orbit element offsets
+    uint64_t c[1 << kOrbitMax];  // orbit element offsets (host planning)
+    uint64_t b[kOrbitMax];       // basis vectors, pivots ascending
     uint32_t piv[kOrbitMax];     // pivot bits, ascending
 };
 
@@ -554,17 +555,32 @@ __device__ __forceinline__ void orbit_pexp_dispatch(uint32_t v, double2 (&a)[1 <
     }
 }
 
+// The basis travels as scalar kernel arguments (a by-value struct argument
+// indexed in the unrolled loops was copied to scratch memory).
 template <int M>
-__global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, uint64_t norbit, OrbitDesc d,
-                                                     const OrbitOp* __restrict__ ops, uint32_t nops) {
+__global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, uint64_t norbit, uint64_t b0,
+                                                     uint64_t b1, uint64_t b2, uint64_t b3, uint32_t p0, uint32_t p1,
+                                                     uint32_t p2, uint32_t p3, const OrbitOp* __restrict__ ops,
+                                                     uint32_t nops) {
     const uint64_t t = (uint64_t)blockIdx.x * kOrbitTPB + threadIdx.x;
     if (t >= norbit) return;
+    const uint64_t bb[4] = {b0, b1, b2, b3};
+    const uint32_t pp[4] = {p0, p1, p2, p3};
     uint64_t j = t;
 #pragma unroll
-    for (int k = 0; k < M; ++k) j = ins0(j, d.piv[k]);
+    for (int k = 0; k < M; ++k) j = ins0(j, pp[k]);
+    uint64_t off[1 << M];
+#pragma unroll
+    for (int i = 0; i < (1 << M); ++i) {
+        uint64_t c = 0;
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+            if ((i >> k) & 1) c ^= bb[k];
+        off[i] = j ^ c;
+    }
     double2 a[1 << M];
 #pragma unroll
-    for (int i = 0; i < (1 << M); ++i) a[i] = s[j ^ d.c[i]];
+    for (int i = 0; i < (1 << M); ++i) a[i] = s[off[i]];
     for (uint32_t o = 0; o < nops; ++o) {
         const OrbitOp op = ops[o];
         const bool pj = __popcll(j & op.z) & 1;
@@ -581,7 +597,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
         }
     }
 #pragma unroll
-    for (int i = 0; i < (1 << M); ++i) s[j ^ d.c[i]] = a[i];
+    for (int i = 0; i < (1 << M); ++i) s[off[i]] = a[i];
 }
 
 // Reduced GF(2) basis with pivot = highest bit; every pivot bit appears in
@@ -701,13 +717,15 @@ template <int M>
 static void launch_orbit_k(double2* s, uint64_t norbit, const OrbitDesc& d, const OrbitOp* ops, uint32_t nops,
                            hipStream_t st) {
     const unsigned blocks = (unsigned)((norbit + kOrbitTPB - 1) / kOrbitTPB);
-    hipLaunchKernelGGL(k_orbit<M>, dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, d, ops, nops);
+    hipLaunchKernelGGL(k_orbit<M>, dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, d.b[0], d.b[1], d.b[2], d.b[3],
+                       d.piv[0], d.piv[1], d.piv[2], d.piv[3], ops, nops);
 }
 
 static int launch_orbit(qfe_state_s* st, int M, const OrbitDesc& d, const OrbitOp* ops, uint32_t nops) {
     Context& cx = ctx();
     const uint64_t norbit = st->N >> M;
     switch (M) {
+        case 0: launch_orbit_k<0>(st->d, norbit, d, ops, nops, cx.stream); break;  // only diagonal exponentials
         case 1: launch_orbit_k<1>(st->d, norbit, d, ops, nops, cx.stream); break;
         case 2: launch_orbit_k<2>(st->d, norbit, d, ops, nops, cx.stream); break;
         case 3: launch_orbit_k<3>(st->d, norbit, d, ops, nops, cx.stream); break;
@@ -726,7 +744,10 @@ static void build_orbit(const Pass& p, const std::vector<HostOp>& ops, OrbitDesc
     std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
     const int M = (int)b.size();
     d = OrbitDesc{};
-    for (int k = 0; k < M; ++k) d.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+    for (int k = 0; k < M; ++k) {
+        d.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+        d.b[k] = b[k];
+    }
     for (int i = 0; i < (1 << M); ++i) {
         uint64_t c = 0;
         for (int k = 0; k < M; ++k)
