@@ -150,6 +150,11 @@ int qfe_apply_pauli_sum(qfe_state in, const qfe_pauli* terms, uint64_t count, qf
  * measure_gradient / measure_gradient3 (uccvqeabc.py:159-162, 242-244,
  * 274-278) without forming K psi. */
 int qfe_cross_expval(qfe_state bra, qfe_state ket, const qfe_pauli* terms, uint64_t count, qfe_c64* out);
+/* Several operators in one launch: out[k] = <bra|O_k|ket>, operators
+ * delimited by op_offsets (the per-operator vdot loops of QITE's build_b and
+ * build_S, qite.py:256-326, and of measure_gradient3). */
+int qfe_cross_expval_many(qfe_state bra, qfe_state ket, const qfe_pauli* terms, const uint64_t* op_offsets,
+                          uint64_t nops, qfe_c64* out);
 
 /* ---- compiled UCC programs (no single reference counterpart) ------------
  * A program holds, resident in HBM, the exponential sequence of a

@@ -28,6 +28,7 @@ from ._core import (  # noqa: F401
     UCCProgram,
     add_gate_to_computer,
     control_gate,
+    cross_expectations,
     device_count,
     find_irrep,
     gate,
@@ -59,3 +60,10 @@ from .system.hamiltonians import (  # noqa: E402,F401
 )
 from .engine.program import compile_ucc_program, ucc_generators  # noqa: E402,F401
 from .engine.serving import ServingEngine  # noqa: E402,F401
+from .abc.algorithm import Algorithm, AnsatzAlgorithm  # noqa: E402,F401
+from .abc.uccvqeabc import UCCVQE  # noqa: E402,F401
+from .ucc.uccnvqe import UCCNVQE  # noqa: E402,F401
+from .ucc.adaptvqe import ADAPTVQE  # noqa: E402,F401
+from .ucc.uccnpqe import UCCNPQE  # noqa: E402,F401
+from .ite.qite import QITE  # noqa: E402,F401
+from .maths.eigsolve import canonical_geig_solve  # noqa: E402,F401

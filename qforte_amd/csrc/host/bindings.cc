@@ -406,6 +406,25 @@ PYBIND11_MODULE(_core, m) {
         return comp;
     }, "gate"_a, "computer"_a);
     m.def("inner_product", &qfa::dot, "a"_a, "b"_a);
+    m.def("cross_expectations", [](const Computer& bra, const Computer& ket, const std::vector<QubitOperator>& ops) {
+        // <bra|O_k|ket> for every Pauli-sum operator O_k, one device launch
+        std::vector<qfe_pauli> terms, one;
+        std::vector<uint64_t> off{0};
+        for (const auto& o : ops) {
+            if (!compile_pauli_sum(o, one)) throw std::invalid_argument("cross_expectations: operators must be Pauli sums");
+            terms.insert(terms.end(), one.begin(), one.end());
+            off.push_back(terms.size());
+        }
+        std::vector<qfe_c64> r(ops.size());
+        {
+            py::gil_scoped_release rel;
+            check(qfe_cross_expval_many(bra.handle(), ket.handle(), terms.data(), off.data(), ops.size(), r.data()),
+                  "cross_expectations");
+        }
+        std::vector<std::complex<double>> out;
+        for (const auto& v : r) out.emplace_back(v.re, v.im);
+        return out;
+    }, "bra"_a, "ket"_a, "operators"_a);
     m.def("norm2", [](const Computer& c) {
         double v = 0.0;
         check(qfe_state_norm2(c.handle(), &v), "norm2");

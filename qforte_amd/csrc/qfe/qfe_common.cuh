@@ -34,6 +34,25 @@ __device__ __forceinline__ uint64_t ins00(uint64_t k, uint32_t lo_b, uint32_t hi
 
 __device__ __forceinline__ double parity_sign(uint64_t v) { return (__popcll(v) & 1) ? -1.0 : 1.0; }
 
+// c_i <- c_{i ^ v}: a butterfly of conditional swaps, one stage per bit of v.
+// Every array index is a compile-time constant (v only selects whether a
+// stage swaps), so the array stays in registers.
+template <int M>
+__device__ __forceinline__ void xor_permute(double2 (&c)[1 << M], uint32_t v) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        if ((v >> k) & 1u) {
+#pragma unroll
+            for (int i = 0; i < (1 << M); ++i) {
+                if (i & (1 << k)) continue;
+                const double2 t = c[i];
+                c[i] = c[i | (1 << k)];
+                c[i | (1 << k)] = t;
+            }
+        }
+    }
+}
+
 // Wave-level sum of a double2 (fixed butterfly order -> deterministic).
 __device__ __forceinline__ double2 wave_sum(double2 v) {
 #pragma unroll

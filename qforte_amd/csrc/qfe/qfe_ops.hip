@@ -529,29 +529,23 @@ struct OrbitDesc {
     uint32_t piv[kOrbitMax];     // pivot bits, ascending
 };
 
-template <int M, int V>
-__device__ __forceinline__ void orbit_pexp_v(double2 (&a)[1 << M], bool pj, uint32_t smask, double c, double sn,
-                                             double2 ph) {
-    constexpr int HB = 31 - __builtin_clz((unsigned)V);
+// exp(i th P) inside an orbit: every element i takes
+//   a_i' = cos a_i + i sin ph (-1)^{s(i ^ v)} a_{i ^ v},
+// the same formula for both members of each pair; the partner array comes
+// from the xor butterfly, so no array index depends on v.
+template <int M>
+__device__ __forceinline__ void orbit_pexp(double2 (&a)[1 << M], uint32_t v, uint32_t sm, double c, double sn,
+                                           double2 ph) {
+    double2 p[1 << M];
+#pragma unroll
+    for (int i = 0; i < (1 << M); ++i) p[i] = a[i];
+    xor_permute<M>(p, v);
+    const double2 iph = make_double2(-ph.y, ph.x);  // i * phase
 #pragma unroll
     for (int i = 0; i < (1 << M); ++i) {
-        if (i & (1 << HB)) continue;
-        const int k = i ^ V;
-        const bool na = pj != (((smask >> i) & 1u) != 0);
-        const bool nb = pj != (((smask >> k) & 1u) != 0);
-        pexp_pair(a[i], a[k], c, sn, ph, na, nb);
-    }
-}
-
-template <int M, int V>
-__device__ __forceinline__ void orbit_pexp_dispatch(uint32_t v, double2 (&a)[1 << M], bool pj, uint32_t smask,
-                                                    double c, double sn, double2 ph) {
-    if constexpr (V < (1 << M)) {
-        if (v == (uint32_t)V) {
-            orbit_pexp_v<M, V>(a, pj, smask, c, sn, ph);
-            return;
-        }
-        orbit_pexp_dispatch<M, V + 1>(v, a, pj, smask, c, sn, ph);
+        const double s = ((sm >> ((uint32_t)i ^ v)) & 1u) ? -sn : sn;
+        const double2 t = cmul(iph, p[i]);
+        a[i] = make_double2(c * a[i].x + s * t.x, c * a[i].y + s * t.y);
     }
 }
 
@@ -593,7 +587,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
                 a[i] = cmul(neg ? fm : fp, a[i]);
             }
         } else {
-            orbit_pexp_dispatch<M, 1>(op.v, a, pj, op.smask, op.c, op.sn, op.ph);
+            orbit_pexp<M>(a, op.v, op.smask ^ (pj ? ~0u : 0u), op.c, op.sn, op.ph);
         }
     }
 #pragma unroll

@@ -203,11 +203,256 @@ int launch_apply_sum_async(const double2* in, double2* out, int n, const PauliGr
     return QFE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Orbit-fused expectation passes.  Groups whose X masks span <= 4 dimensions
+// over GF(2) are evaluated together: every thread loads the 2^M amplitudes
+// of its orbit {j ^ c_i} once and accumulates, for each group g with mask
+// v_g in orbit coordinates, sum_i conj(bra_i) ket_{i^v} f_g(i^v).  One read
+// pass of the state then serves up to 16 groups.
+// ---------------------------------------------------------------------------
+constexpr int kEvOrbitMax = 4;
+constexpr int kEvOrbitGroups = 16;
+
+struct OrbitGroup {
+    uint32_t v;      // X mask in orbit coordinates
+    uint32_t first;  // first term (into the OrbitTerm array)
+    uint32_t count;
+    uint32_t slot;   // output row
+};
+
+struct OrbitTerm {
+    uint64_t z;
+    uint32_t smask;  // bit i = parity(c_i & z)
+    uint32_t pad;
+    double2 c;       // coefficient * phase
+};
+
+
+// SAME: bra and ket are one state (plain expectation value): load it once.
+template <int M, bool SAME>
+__global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restrict__ bra,
+                                                         const double2* __restrict__ ket, uint64_t norbit,
+                                                         uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
+                                                         uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
+                                                         const OrbitGroup* __restrict__ groups, uint32_t ngroups,
+                                                         const OrbitTerm* __restrict__ terms,
+                                                         double2* __restrict__ partial, uint32_t nb) {
+    __shared__ double2 acc_s[kEvOrbitGroups][kEvTPB];
+    __shared__ double2 lds[16];
+    const uint64_t bb[4] = {b0, b1, b2, b3};
+    const uint32_t pp[4] = {p0, p1, p2, p3};
+    for (uint32_t g = 0; g < ngroups; ++g) acc_s[g][threadIdx.x] = make_double2(0.0, 0.0);
+    for (uint64_t t = (uint64_t)blockIdx.x * kEvTPB + threadIdx.x; t < norbit; t += (uint64_t)gridDim.x * kEvTPB) {
+        uint64_t j = t;
+#pragma unroll
+        for (int k = 0; k < M; ++k) j = ins0(j, pp[k]);
+        double2 a[1 << M], b[1 << M];
+#pragma unroll
+        for (int i = 0; i < (1 << M); ++i) {
+            uint64_t c = 0;
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+                if ((i >> k) & 1) c ^= bb[k];
+            a[i] = ket[j ^ c];
+            if constexpr (!SAME) b[i] = bra[j ^ c];
+        }
+        if constexpr (SAME) {
+#pragma unroll
+            for (int i = 0; i < (1 << M); ++i) b[i] = a[i];
+        }
+        for (uint32_t g = 0; g < ngroups; ++g) {
+            const OrbitGroup G = groups[g];
+            // p_i = conj(bra_i) ket_{i ^ v}
+            double2 p[1 << M];
+#pragma unroll
+            for (int i = 0; i < (1 << M); ++i) p[i] = a[i];
+            xor_permute<M>(p, G.v);
+#pragma unroll
+            for (int i = 0; i < (1 << M); ++i) p[i] = cconjmul(b[i], p[i]);
+            double2 acc = make_double2(0.0, 0.0);
+            for (uint32_t tt = 0; tt < G.count; ++tt) {
+                const OrbitTerm T = terms[G.first + tt];
+                const uint32_t flip = (__popcll(j & T.z) & 1) ? ~0u : 0u;
+                const uint32_t sm = T.smask ^ flip;
+                // w = sum_i (-1)^{s(i ^ v)} p_i: the sign belongs to the ket element i ^ v
+                double2 w = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int i = 0; i < (1 << M); ++i) {
+                    if ((sm >> ((uint32_t)i ^ G.v)) & 1u) {
+                        w.x -= p[i].x;
+                        w.y -= p[i].y;
+                    } else {
+                        w.x += p[i].x;
+                        w.y += p[i].y;
+                    }
+                }
+                acc = cadd(acc, cmul(T.c, w));
+            }
+            acc_s[g][threadIdx.x] = cadd(acc_s[g][threadIdx.x], acc);
+        }
+    }
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        const double2 s = block_sum(acc_s[g][threadIdx.x], lds);
+        if (threadIdx.x == 0) partial[(uint64_t)groups[g].slot * nb + blockIdx.x] = s;
+    }
+}
+
+struct EvPass {
+    std::vector<uint32_t> groups;  // indices into the caller's group list
+    std::vector<uint64_t> basis;   // reduced basis, pivot = highest bit
+};
+
+// Pack groups into passes whose X masks span <= 4 dimensions (first fit).
+static void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes) {
+    std::vector<char> used(groups.size(), 0);
+    for (size_t s = 0; s < groups.size(); ++s) {
+        if (used[s]) continue;
+        EvPass p;
+        for (size_t g = s; g < groups.size() && p.groups.size() < (size_t)kEvOrbitGroups; ++g) {
+            if (used[g]) continue;
+            // reduce x against the basis; accept if it stays within kEvOrbitMax
+            uint64_t r = groups[g].x;
+            for (uint64_t v : p.basis)
+                if ((r >> (63 - __builtin_clzll(v))) & 1ull) r ^= v;
+            if (r) {
+                if ((int)p.basis.size() >= kEvOrbitMax) continue;
+                const int piv = 63 - __builtin_clzll(r);
+                for (uint64_t& v : p.basis)
+                    if ((v >> piv) & 1ull) v ^= r;
+                p.basis.push_back(r);
+            }
+            used[g] = 1;
+            p.groups.push_back((uint32_t)g);
+        }
+        passes.push_back(std::move(p));
+    }
+}
+
+template <int M>
+static void launch_orbit_expval_k(const double2* bra, const double2* ket, uint64_t norbit, const uint64_t* b,
+                                  const uint32_t* piv, const OrbitGroup* dg, uint32_t ng, const OrbitTerm* dt,
+                                  double2* partial, uint32_t nb, hipStream_t st) {
+    if (bra == ket) {
+        hipLaunchKernelGGL((k_orbit_expval<M, true>), dim3(nb), dim3(kEvTPB), 0, st, bra, ket, norbit, b[0], b[1],
+                           b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    } else {
+        hipLaunchKernelGGL((k_orbit_expval<M, false>), dim3(nb), dim3(kEvTPB), 0, st, bra, ket, norbit, b[0], b[1],
+                           b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    }
+}
+
+static int expval_groups_orbit(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
+                               const std::vector<uint64_t>& zs, const std::vector<double2>& cs,
+                               double2* out_per_group) {
+    Context& cx = ctx();
+    const size_t G = groups.size();
+    const uint64_t N = 1ull << n;
+    std::vector<EvPass> passes;
+    plan_expval(groups, passes);
+    uint64_t nb64 = N / 4096;
+    if (nb64 < 1) nb64 = 1;
+    if (nb64 > 2048) nb64 = 2048;
+    const uint32_t nb = (uint32_t)nb64;
+    // host tables: per pass its OrbitGroup run and OrbitTerm run
+    std::vector<OrbitGroup> og;
+    std::vector<OrbitTerm> ot;
+    struct Launch {
+        int M;
+        uint64_t b[4];
+        uint32_t piv[4];
+        size_t g0, ng;
+    };
+    std::vector<Launch> launches;
+    for (const EvPass& p : passes) {
+        std::vector<uint64_t> b = p.basis;
+        std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
+        Launch L{};
+        L.M = (int)b.size();
+        for (int k = 0; k < L.M; ++k) {
+            L.b[k] = b[k];
+            L.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+        }
+        uint64_t coff[1 << kEvOrbitMax];
+        for (int i = 0; i < (1 << L.M); ++i) {
+            uint64_t c = 0;
+            for (int k = 0; k < L.M; ++k)
+                if ((i >> k) & 1) c ^= b[k];
+            coff[i] = c;
+        }
+        L.g0 = og.size();
+        L.ng = p.groups.size();
+        for (uint32_t gi : p.groups) {
+            const PauliGroup& pg = groups[gi];
+            OrbitGroup o{};
+            for (int k = 0; k < L.M; ++k)
+                if ((pg.x >> L.piv[k]) & 1ull) o.v |= 1u << k;
+            o.first = (uint32_t)ot.size();
+            o.count = pg.count;
+            o.slot = gi;
+            og.push_back(o);
+            for (uint32_t t = 0; t < pg.count; ++t) {
+                OrbitTerm T{};
+                T.z = zs[pg.first + t];
+                for (int i = 0; i < (1 << L.M); ++i)
+                    if (__builtin_popcountll(coff[i] & T.z) & 1) T.smask |= 1u << i;
+                T.c = cs[pg.first + t];
+                ot.push_back(T);
+            }
+        }
+        launches.push_back(L);
+    }
+    const size_t b_g = align256(og.size() * sizeof(OrbitGroup));
+    const size_t b_t = align256(ot.size() * sizeof(OrbitTerm));
+    const size_t b_part = align256(G * (size_t)nb * sizeof(double2));
+    const size_t b_out = align256(G * sizeof(double2));
+    void* scr = nullptr;
+    int rc = scratch_reserve(b_g + b_t + b_part + b_out, &scr);
+    if (rc) return rc;
+    char* base = static_cast<char*>(scr);
+    char* host = nullptr;
+    rc = staging_begin(b_g + b_t, reinterpret_cast<void**>(&host));
+    if (rc) return rc;
+    std::memcpy(host, og.data(), og.size() * sizeof(OrbitGroup));
+    std::memcpy(host + b_g, ot.data(), ot.size() * sizeof(OrbitTerm));
+    rc = staging_commit(base, b_g + b_t);
+    if (rc) return rc;
+    const OrbitGroup* dg = reinterpret_cast<const OrbitGroup*>(base);
+    const OrbitTerm* dt = reinterpret_cast<const OrbitTerm*>(base + b_g);
+    double2* partial = reinterpret_cast<double2*>(base + b_g + b_t);
+    double2* dout = reinterpret_cast<double2*>(base + b_g + b_t + b_part);
+    for (const Launch& L : launches) {
+        const uint64_t norbit = N >> L.M;
+        const OrbitGroup* g = dg + L.g0;
+        const uint32_t ng = (uint32_t)L.ng;
+        switch (L.M) {
+            case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
+            case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
+            case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
+            case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
+            case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
+            default: return fail(QFE_ERR_STATE, "internal: expectation orbit dimension out of range");
+        }
+        QFE_HIP(hipGetLastError());
+        cx.counters.reduce++;
+        cx.counters.bytes += (bra == ket ? 16.0 : 32.0) * (double)N;
+    }
+    hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)G), dim3(kEvTPB), 0, cx.stream, partial, nb, dout);
+    QFE_HIP(hipGetLastError());
+    void* pin = nullptr;
+    rc = pinned_reserve(G * sizeof(double2), &pin);
+    if (rc) return rc;
+    QFE_HIP(hipMemcpyAsync(pin, dout, G * sizeof(double2), hipMemcpyDeviceToHost, cx.stream));
+    QFE_HIP(hipStreamSynchronize(cx.stream));
+    std::memcpy(out_per_group, pin, G * sizeof(double2));
+    return QFE_OK;
+}
+
 int expval_groups(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
                   const std::vector<uint64_t>& zs, const std::vector<double2>& cs, double2* out_per_group) {
     Context& cx = ctx();
     const size_t G = groups.size();
     if (G == 0) return QFE_OK;
+    if (n >= kEvOrbitMax + 8) return expval_groups_orbit(bra, ket, n, groups, zs, cs, out_per_group);
     const uint32_t nchunk = chunks_for(n, G);
     const size_t b_part = align256(G * nchunk * sizeof(double2));
     const size_t b_out = align256(G * sizeof(double2));
@@ -304,7 +549,14 @@ int qfe_expval(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out
 }
 
 int qfe_expval_many(qfe_state s, const qfe_pauli* terms, const uint64_t* op_offsets, uint64_t nops, qfe_c64* out) {
-    if (!valid(s) || (nops && (!op_offsets || !out))) return fail(QFE_ERR_ARG, "qfe_expval_many: invalid argument");
+    return qfe_cross_expval_many(s, s, terms, op_offsets, nops, out);
+}
+
+int qfe_cross_expval_many(qfe_state bra, qfe_state s, const qfe_pauli* terms, const uint64_t* op_offsets,
+                          uint64_t nops, qfe_c64* out) {
+    if (!valid(s) || !valid(bra) || (nops && (!op_offsets || !out)))
+        return fail(QFE_ERR_ARG, "qfe_cross_expval_many: invalid argument");
+    if (bra->n != s->n) return fail(QFE_ERR_ARG, "qfe_cross_expval_many: qubit counts differ");
     if (nops == 0) return QFE_OK;
     const uint64_t total = op_offsets[nops];
     int rc = check_terms(terms, total, s->n);
@@ -327,7 +579,7 @@ int qfe_expval_many(qfe_state s, const qfe_pauli* terms, const uint64_t* op_offs
         cs.insert(cs.end(), c1.begin(), c1.end());
     }
     std::vector<double2> per(groups.size());
-    rc = expval_groups(s->d, s->d, s->n, groups, zs, cs, per.data());
+    rc = expval_groups(bra->d, s->d, s->n, groups, zs, cs, per.data());
     if (rc) return rc;
     for (uint64_t k = 0; k < nops; ++k) out[k] = qfe_c64{0.0, 0.0};
     for (size_t g = 0; g < groups.size(); ++g) {
