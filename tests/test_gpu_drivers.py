@@ -1,0 +1,107 @@
+"""The restated drivers on the engine reproduce the reference's integration
+tests: FCI energies of He/cc-pVDZ (UCCSD-VQE, BFGS and Jacobi; UCCSD-PQE),
+H4/STO-6G (ADAPT-VQE), H2 active space (PQE) and H2 QITE
+(reference tests/test_uccsd.py, test_adapt_vqe.py, test_external_builder.py,
+test_qite.py, test_computer_init.py)."""
+
+import os
+
+import numpy as np
+import pytest
+from pytest import approx
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def he(qf):
+    return qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "He-ccpvdz.json"))
+
+
+def test_he_uccsd_vqe(qf):
+    # tests/test_uccsd.py:14-29
+    alg = qf.UCCNVQE(he(qf))
+    alg.run(pool_type="SD", use_analytic_grad=True)
+    assert alg.get_gs_energy() == approx(-2.887594831090938, abs=1.0e-10)
+
+
+def test_he_uccsd_vqe_gate_path(qf):
+    # the same optimisation through the gate-level circuit path of Computer
+    alg = qf.UCCNVQE(he(qf), use_program=False)
+    alg.run(pool_type="SD", use_analytic_grad=True)
+    assert alg.get_gs_energy() == approx(-2.887594831090938, abs=1.0e-10)
+
+
+def test_he_uccsd_vqe_jacobi(qf):
+    # tests/test_uccsd.py:31-47
+    alg = qf.UCCNVQE(he(qf))
+    alg.run(pool_type="SD", use_analytic_grad=True, optimizer="jacobi")
+    assert alg.get_gs_energy() == approx(-2.887594831090938, abs=1.0e-11)
+
+
+def test_he_uccsd_pqe(qf):
+    # tests/test_uccsd.py:69-84
+    alg = qf.UCCNPQE(he(qf))
+    alg.run(pool_type="SD", opt_thresh=1.0e-7)
+    assert alg.get_gs_energy() == approx(-2.887594831090938, abs=1.0e-11)
+
+
+def test_h2_active_space_pqe(qf):
+    # tests/test_external_builder.py:9-29
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "as_ints.json"))
+    alg = qf.UCCNPQE(mol)
+    alg.run(pool_type="SD", opt_thresh=1.0e-7)
+    assert alg.get_gs_energy() == approx(-1.129181390402433, abs=1.0e-12)
+
+
+def test_h4_adapt_vqe(qf):
+    # tests/test_adapt_vqe.py:12-40
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    alg = qf.ADAPTVQE(mol, print_summary_file=False)
+    alg.run(adapt_maxiter=20, avqe_thresh=1.0e-4, opt_thresh=1.0e-5, use_analytic_grad=True, pool_type="SDTQ")
+    assert alg.get_gs_energy() == approx(-2.1628978832666865, abs=5.0e-11)
+
+
+def test_h4_vqe_from_computer_reference(qf):
+    # tests/test_computer_init.py:13-30: a Computer as the reference state,
+    # analytic and finite-difference gradients reach the same minimum
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    n = len(mol.hf_reference)
+    comp = qf.Computer(n)
+    v = np.zeros(2**n)
+    v[int("00001111", 2)] = 1
+    v[int("00110011", 2)] = 0.2
+    v[int("00111100", 2)] = 0.1
+    v[int("11001100", 2)] = 0.04
+    comp.set_coeff_vec(v / np.linalg.norm(v))
+    e = []
+    for analytic in (True, False):
+        alg = qf.UCCNVQE(mol, reference=comp, state_prep_type="computer")
+        alg.run(pool_type="SD", use_analytic_grad=analytic)
+        e.append(alg.get_gs_energy())
+    assert e[0] == approx(e[1], abs=1.0e-8)
+    # ADAPT-GSD from the HF determinant and from the Computer reach FCI
+    hf = qf.ADAPTVQE(mol)
+    hf.run(use_analytic_grad=True, pool_type="GSD", avqe_thresh=1e-5)
+    cp = qf.ADAPTVQE(mol, reference=comp, state_prep_type="computer")
+    cp.run(use_analytic_grad=True, pool_type="GSD", avqe_thresh=1e-5)
+    assert hf.get_gs_energy() == approx(cp.get_gs_energy(), abs=1.0e-8)
+    assert hf.get_gs_energy() == approx(-2.162897881184882, abs=1.0e-8)
+
+
+def test_h2_qite(qf):
+    # tests/test_qite.py:8-81: H2 at 1.5 A in STO-3G, Hamiltonian data given there
+    coef = [-0.4917857774144603, 0.09345649662931771, 0.09345649662931771, -0.0356448161226769,
+            -0.0356448161226769, 0.1381758457453024, 0.05738398402634884, -0.0573839840263488,
+            -0.0573839840263488, 0.05738398402634884, 0.08253705485911705, 0.13992103888546592,
+            0.13992103888546592, 0.08253705485911705, 0.1458551902800438]
+    circs = ["", "Z_0", "Z_1", "Z_2", "Z_3", "Z_0 Z_1", "Y_0 X_1 X_2 Y_3", "X_0 X_1 Y_2 Y_3", "Y_0 Y_1 X_2 X_3",
+             "X_0 Y_1 Y_2 X_3", "Z_0 Z_2", "Z_0 Z_3", "Z_1 Z_2", "Z_1 Z_3", "Z_2 Z_3"]
+    H = qf.QubitOperator()
+    for c, s in zip(coef, circs):
+        H.add(c, qf.build_circuit(s) if s else qf.Circuit())
+    mol = qf.Molecule()
+    mol.hamiltonian = H
+    alg = qf.QITE(mol, reference=[1, 1, 0, 0])
+    alg.run(beta=18.0, do_lanczos=True, lanczos_gap=49)
+    assert alg.get_gs_energy() == approx(-0.9981493534, abs=1.0e-10)
