@@ -136,6 +136,24 @@ int launch_apply_sum_async(const double2* in, double2* out, int n, const PauliGr
 // program introspection for the serving engine
 int program_nqubit(qfe_program p);
 uint64_t program_nparam(qfe_program p);
+struct ProgramView {
+    int n;
+    uint64_t ref;
+    uint64_t nparam;
+    const std::vector<qfe_pauli>* gen_terms;
+    const std::vector<uint64_t>* gen_off;
+    const std::vector<int32_t>* gparam;
+    const std::vector<double>* gscale;
+    const std::vector<qfe_pauli>* ham_terms;
+};
+ProgramView program_view(qfe_program p);
+
+// Packing of X-mask groups into orbit passes (span of the masks <= 4 dims).
+struct EvPass {
+    std::vector<uint32_t> groups;  // indices into the group list
+    std::vector<uint64_t> basis;   // reduced basis, pivot = highest bit
+};
+void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes);
 
 // device helpers exposed across translation units
 int launch_zero(double2* d, uint64_t N, hipStream_t st);

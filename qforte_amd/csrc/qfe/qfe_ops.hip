@@ -805,10 +805,60 @@ static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) 
     return QFE_OK;
 }
 
+// Adjacent one-qubit ops on the same target under the same condition become
+// one 2x2 (U = U_second * U_first): Ry then Rz on a qubit is one sweep op.
+static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
+    auto as2x2 = [](const DevOp& o, double2 m[4]) {
+        if (o.kind == K_U1) {
+            for (int i = 0; i < 4; ++i) m[i] = o.m[i];
+        } else {
+            m[0] = o.m[0];
+            m[1] = make_double2(0.0, 0.0);
+            m[2] = make_double2(0.0, 0.0);
+            m[3] = o.m[1];
+        }
+    };
+    auto mul = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    auto add = [](double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); };
+    std::vector<HostOp> out;
+    out.reserve(ops.size());
+    for (const HostOp& h : ops) {
+        const bool one_q = h.op.kind == K_U1 || h.op.kind == K_DIAG1;
+        if (one_q && !out.empty()) {
+            HostOp& p = out.back();
+            const bool prev_one_q = p.op.kind == K_U1 || p.op.kind == K_DIAG1;
+            if (prev_one_q && p.op.t == h.op.t && p.op.cmask == h.op.cmask && p.op.cval == h.op.cval) {
+                double2 a[4], b[4], r[4];
+                as2x2(p.op, a);
+                as2x2(h.op, b);
+                r[0] = add(mul(b[0], a[0]), mul(b[1], a[2]));
+                r[1] = add(mul(b[0], a[1]), mul(b[1], a[3]));
+                r[2] = add(mul(b[2], a[0]), mul(b[3], a[2]));
+                r[3] = add(mul(b[2], a[1]), mul(b[3], a[3]));
+                const bool diag = r[1].x == 0.0 && r[1].y == 0.0 && r[2].x == 0.0 && r[2].y == 0.0;
+                if (diag) {
+                    p.op.kind = K_DIAG1;
+                    p.op.m[0] = r[0];
+                    p.op.m[1] = r[3];
+                    p.mix = 0;
+                } else {
+                    p.op.kind = K_U1;
+                    for (int i = 0; i < 4; ++i) p.op.m[i] = r[i];
+                    p.mix = 1ull << p.op.t;
+                }
+                continue;
+            }
+        }
+        out.push_back(h);
+    }
+    ops.swap(out);
+}
+
 int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     if (ops.empty()) return QFE_OK;
     Context& cx = ctx();
     const int n = st->n;
+    merge_single_qubit_runs(ops);
     std::vector<Pass> passes;
     plan(ops, n, cx.tuning, passes);
 

@@ -297,13 +297,8 @@ __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restri
     }
 }
 
-struct EvPass {
-    std::vector<uint32_t> groups;  // indices into the caller's group list
-    std::vector<uint64_t> basis;   // reduced basis, pivot = highest bit
-};
-
 // Pack groups into passes whose X masks span <= 4 dimensions (first fit).
-static void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes) {
+void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes) {
     std::vector<char> used(groups.size(), 0);
     for (size_t s = 0; s < groups.size(); ++s) {
         if (used[s]) continue;

@@ -181,6 +181,9 @@ static int energy_of(qfe_program p, qfe_state s, double* energy) {
 
 int program_nqubit(qfe_program p) { return p->n; }
 uint64_t program_nparam(qfe_program p) { return p->nparam; }
+ProgramView program_view(qfe_program p) {
+    return ProgramView{p->n, p->ref, p->nparam, &p->gen_terms, &p->gen_off, &p->gparam, &p->gscale, &p->ham_terms};
+}
 
 }  // namespace qfe
 
