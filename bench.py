@@ -31,6 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per k_orbit<4> launch at 30 qubits from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by the same counter's reading on k_scale (a plain
+# read-write sweep of known size); filled in from profiles/ once measured.
+TRAFFIC_PER_LAUNCH = None
 METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
 
 
@@ -45,7 +49,22 @@ def build_workload(qf, W, n, nlayers, nexp, nterms):
     return layers, strings, terms, layer_circ, U, H
 
 
-def time_dominant_kernel(qf, W, qc, n, reps=40):
+def time_exp_stage(qf, qc, U):
+    """Average launch duration of the passes of the exponential stage (the
+    dominant kernel: register-orbit passes, one read-write sweep of the state
+    each), HIP events on the engine stream around one application of U."""
+    qf.synchronize()
+    qf.reset_counters()
+    t = qf.DeviceTimer()
+    t.start()
+    qc.apply_circuit(U)
+    ms = t.stop()
+    c = qf.get_counters()
+    passes = c["fused_passes"] + c["single_passes"]
+    return ms / passes, passes
+
+
+def time_single_sweep(qf, W, qc, n, reps=40):
     """Average duration of the single Pauli-exponential sweep (one op per
     pass) over `reps` back-to-back launches, HIP events on the engine stream."""
     strings = W.random_pauli_strings(n, reps, seed=104)
@@ -118,6 +137,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true", help="skip the dominant-kernel timing")
     ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the exponential stage and a calibration sweep (for hardware-counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -141,6 +162,18 @@ def main():
           file=sys.stderr, flush=True)
     layers, strings, terms, layer_circ, U, H = build_workload(qf, W, n, args.layers, args.nexp, args.nterms)
     qc = qf.Computer(n)
+
+    if args.roofline_only:
+        # short run for hardware-counter passes: the exponential stage once,
+        # then two plain read-write sweeps (k_scale) as the calibration kernel
+        qc.apply_circuit(U)
+        avg_ms, passes = time_exp_stage(qf, qc, U)
+        qc.apply_constant(1.0)
+        qc.apply_constant(1.0)
+        qf.synchronize()
+        print(json.dumps({"exp_stage_passes": passes, "avg_launch_ms": round(avg_ms, 4),
+                          "state_bytes": 16 * (1 << n)}))
+        return
 
     def step():
         qc.reset()
@@ -171,17 +204,21 @@ def main():
 
     roofline = None
     if not args.no_roofline:
-        avg_ms = time_dominant_kernel(qf, W, qc, n)
+        avg_ms, passes = time_exp_stage(qf, qc, U)
         achieved = 2 * S / (avg_ms * 1e-3) / 1e9
+        single_ms = time_single_sweep(qf, W, qc, n)
         roofline = {
-            "kernel": "k_pexp (single Pauli-exponential sweep, 2S = %.2f GB per launch)" % (2 * S / 1e9),
+            "kernel": ("k_orbit<4> (register-orbit pass of the exponential stage: %d passes for %d exponentials, "
+                       "2S = %.2f GB per launch)" % (passes, len(strings), 2 * S / 1e9)),
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": TRAFFIC_PER_LAUNCH,
             "avg_launch_ms": round(avg_ms, 4),
+            "single_exponential_sweep": {"kernel": "k_pexp", "avg_launch_ms": round(single_ms, 4),
+                                         "achieved": round(2 * S / (single_ms * 1e-3) / 1e9, 1)},
         }
 
     cpu = None
