@@ -32,9 +32,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per k_orbit<4> launch at 30 qubits from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, each scaled by the same counter's reading on k_scale (a plain
-# read-write sweep of known size); filled in from profiles/ once measured.
-TRAFFIC_PER_LAUNCH = None
+# WRITE_SIZE (KiB per dispatch: 8388654 and 16777235), each scaled by the same
+# counter's reading on k_scale, a plain sweep that reads and writes exactly
+# 16 GiB (factors 2.0000 and 1.0000): 33554543 KiB.
+TRAFFIC_PER_LAUNCH = {30: 33554543 * 1024}
 METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
 
 
@@ -215,7 +216,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": TRAFFIC_PER_LAUNCH,
+            "traffic": TRAFFIC_PER_LAUNCH.get(n),
             "avg_launch_ms": round(avg_ms, 4),
             "single_exponential_sweep": {"kernel": "k_pexp", "avg_launch_ms": round(single_ms, 4),
                                          "achieved": round(2 * S / (single_ms * 1e-3) / 1e9, 1)},

@@ -186,18 +186,57 @@ int qfe_program_gradient(qfe_program p, const double* params, double* grad, doub
 int qfe_program_pool_gradients(qfe_program p, const double* params, const qfe_pauli* pool_terms,
                                const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 
+/* ---- determinant-sector templates --------------------------------------
+ * A template compiles one size class of UCC energy evaluations: a
+ * disentangled ansatz on the determinant ref_index, and a Hamiltonian whose
+ * coefficients are linear in per-request inputs.  Replaces, per request,
+ * UCC.ansatz_circuit + trotterize + measure_energy (ansatz.py:25-82,
+ * algorithm.py:271-334, 458-480) with the same mathematics.
+ *
+ * The generators (gen_terms split by gen_offsets, applied in order, each
+ * as exp(scale * params[param] * sum_j coeff_j * phase_j * P_j), term by
+ * term) must conserve the particle numbers on even bits (alpha) and odd
+ * bits (beta): every run of consecutive terms with one X mask and mutually
+ * commuting strings must map sector determinants to sector determinants.
+ * The state is then kept on the determinants of the reference's
+ * (N_alpha, N_beta) sector only, and each run is applied exactly as the
+ * product of its commuting exponentials: a rotation of determinant pairs.
+ * QFE_ERR_ARG if a run does not conserve the sector.
+ *
+ * Hamiltonian term k has masks (x, z) and phase from ham_terms[k]; its
+ * coefficient is phase_k * sum_{e = map_offsets[k]}^{map_offsets[k+1]-1}
+ * map_value[e] * inputs[map_index[e]] with per-request real inputs, or
+ * phase_k * ham_terms[k].coeff when map_offsets is NULL (ninputs = 0). */
+typedef struct qfe_template_s* qfe_template;
+int qfe_template_create(int nqubit, uint64_t ref_index, const qfe_pauli* gen_terms, const uint64_t* gen_offsets,
+                        uint64_t ngen, const int32_t* gen_param, const double* gen_scale, uint64_t nparam,
+                        const qfe_pauli* ham_terms, uint64_t nham, const uint64_t* map_offsets,
+                        const uint32_t* map_index, const qfe_c64* map_value, uint64_t ninputs, qfe_template* out);
+int qfe_template_destroy(qfe_template t);
+/* Sector dimension and device bytes one request of this template occupies. */
+int qfe_template_info(qfe_template t, uint64_t* sector_dim, uint64_t* request_bytes);
+/* One request in one launch (no batching): E = <ref|U^dag H U|ref>. */
+int qfe_template_energy(qfe_template t, const double* params, const double* inputs, double* energy);
+/* U|ref> scattered into a full 2^n state (for checks against the full-space path). */
+int qfe_template_prepare(qfe_template t, const double* params, qfe_state out);
+
 /* ---- serving: continuous batching of UCC energy requests ----------------
  * No reference counterpart: the reference evaluates one energy per Python
- * call.  A batch engine admits requests (a compiled program plus its
- * amplitudes) while its HBM budget allows, advances every admitted request
- * together, and hands results back as they complete. */
+ * call.  Requests (a template, amplitudes and Hamiltonian inputs) queue on
+ * the host; each engine step admits queued requests into a launch while the
+ * launch's device footprint stays within its share of hbm_budget_bytes and
+ * max_active, runs every admitted request (of any template and size) in one
+ * kernel, one workgroup per request, on one of a few streams, so launches
+ * overlap on the device while later arrivals queue for the next launch. */
 int qfe_batch_create(uint64_t hbm_budget_bytes, int max_active, qfe_batch* out);
 int qfe_batch_destroy(qfe_batch b);
-/* Queue a request; returns its id.  The program must outlive the request. */
-int qfe_batch_submit(qfe_batch b, qfe_program p, const double* params, uint64_t* request_id);
-/* Run the engine until at least one request completes or nothing is
- * pending; completed results are returned (ids, energies, up to cap). */
+/* Queue a request; params and inputs are copied.  The template must outlive it. */
+int qfe_batch_submit(qfe_batch b, qfe_template t, const double* params, const double* inputs, uint64_t* request_id);
+/* Launch what can be admitted; return completed results (ids, energies, up to
+ * cap).  Blocks for the oldest launch only when nothing else can progress. */
 int qfe_batch_step(qfe_batch b, uint64_t* done_ids, double* done_energies, uint64_t cap, uint64_t* ndone);
+/* Non-blocking form: launch what can be admitted and return what is done. */
+int qfe_batch_poll(qfe_batch b, uint64_t* done_ids, double* done_energies, uint64_t cap, uint64_t* ndone);
 int qfe_batch_pending(qfe_batch b, uint64_t* queued, uint64_t* active);
 
 #ifdef __cplusplus

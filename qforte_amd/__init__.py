@@ -20,11 +20,13 @@ from ._core import (  # noqa: F401
     Computer,
     DeviceTimer,
     Gate,
+    HamiltonianMap,
     QubitBasis,
     QubitOperator,
     QubitOpPool,
     SQOperator,
     SQOpPool,
+    SectorTemplate,
     UCCProgram,
     add_gate_to_computer,
     control_gate,
@@ -58,7 +60,7 @@ from .system.hamiltonians import (  # noqa: E402,F401
     synthetic_integrals,
     synthetic_molecule,
 )
-from .engine.program import compile_ucc_program, ucc_generators  # noqa: E402,F401
+from .engine.program import compile_ucc_program, compile_ucc_template, ucc_generators  # noqa: E402,F401
 from .engine.serving import ServingEngine  # noqa: E402,F401
 from .abc.algorithm import Algorithm, AnsatzAlgorithm  # noqa: E402,F401
 from .abc.uccvqeabc import UCCVQE  # noqa: E402,F401

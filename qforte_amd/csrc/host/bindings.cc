@@ -120,26 +120,131 @@ class UCCProgram {
     qfe_program p_ = nullptr;
 };
 
+// Determinant-sector template: one size class of UCC energy evaluations.
+class SectorTemplate {
+  public:
+    SectorTemplate(int nqubit, uint64_t ref_index, const std::vector<QubitOperator>& generators,
+                   const std::vector<int>& param_index, const std::vector<double>& scale, size_t nparam,
+                   const QubitOperator& hamiltonian)
+        : n_(nqubit), nparam_(nparam) {
+        std::vector<qfe_pauli> gen, ham;
+        std::vector<uint64_t> off;
+        std::vector<int32_t> pi;
+        compile_generators(generators, param_index, scale, gen, off, pi);
+        if (!compile_pauli_sum(hamiltonian, ham))
+            throw std::invalid_argument("SectorTemplate: the Hamiltonian must be a sum of Pauli strings");
+        check(qfe_template_create(nqubit, ref_index, gen.data(), off.data(), generators.size(), pi.data(), scale.data(),
+                                  nparam, ham.data(), ham.size(), nullptr, nullptr, nullptr, 0, &t_),
+              "SectorTemplate");
+    }
+    SectorTemplate(int nqubit, uint64_t ref_index, const std::vector<QubitOperator>& generators,
+                   const std::vector<int>& param_index, const std::vector<double>& scale, size_t nparam,
+                   const HamiltonianMap& map)
+        : n_(nqubit), nparam_(nparam), ninputs_(map.ninputs) {
+        std::vector<qfe_pauli> gen, ham(map.x.size());
+        std::vector<uint64_t> off;
+        std::vector<int32_t> pi;
+        compile_generators(generators, param_index, scale, gen, off, pi);
+        for (size_t k = 0; k < ham.size(); ++k) ham[k] = qfe_pauli{map.x[k], map.z[k], {1.0, 0.0}, {0.0, 0.0}};
+        std::vector<qfe_c64> val(map.value.size());
+        for (size_t e = 0; e < val.size(); ++e) val[e] = qfe_c64{map.value[e], 0.0};
+        check(qfe_template_create(nqubit, ref_index, gen.data(), off.data(), generators.size(), pi.data(), scale.data(),
+                                  nparam, ham.data(), ham.size(), map.offsets.data(), map.index.data(), val.data(),
+                                  map.ninputs, &t_),
+              "SectorTemplate");
+    }
+    ~SectorTemplate() {
+        if (t_) qfe_template_destroy(t_);
+    }
+    SectorTemplate(const SectorTemplate&) = delete;
+    SectorTemplate& operator=(const SectorTemplate&) = delete;
+
+    double energy(const std::vector<double>& t, const std::vector<double>& inputs) {
+        need(t, inputs);
+        double e = 0.0;
+        py::gil_scoped_release rel;
+        check(qfe_template_energy(t_, t.data(), inputs.data(), &e), "SectorTemplate.energy");
+        return e;
+    }
+    void prepare(const std::vector<double>& t, Computer& out) {
+        if (t.size() != nparam_) throw std::invalid_argument("SectorTemplate: wrong number of amplitudes");
+        py::gil_scoped_release rel;
+        check(qfe_template_prepare(t_, t.data(), out.handle()), "SectorTemplate.prepare");
+    }
+    uint64_t dim() const {
+        uint64_t d = 0;
+        qfe_template_info(t_, &d, nullptr);
+        return d;
+    }
+    uint64_t request_bytes() const {
+        uint64_t b = 0;
+        qfe_template_info(t_, nullptr, &b);
+        return b;
+    }
+    int nqubit() const { return n_; }
+    size_t nparam() const { return nparam_; }
+    size_t ninputs() const { return ninputs_; }
+    qfe_template handle() const { return t_; }
+    void need(const std::vector<double>& t, const std::vector<double>& inputs) const {
+        if (t.size() != nparam_) throw std::invalid_argument("SectorTemplate: expected " + std::to_string(nparam_) + " amplitudes");
+        if (inputs.size() != ninputs_)
+            throw std::invalid_argument("SectorTemplate: expected " + std::to_string(ninputs_) + " Hamiltonian inputs");
+    }
+
+  private:
+    static void compile_generators(const std::vector<QubitOperator>& generators, const std::vector<int>& param_index,
+                                   const std::vector<double>& scale, std::vector<qfe_pauli>& terms,
+                                   std::vector<uint64_t>& off, std::vector<int32_t>& pi) {
+        if (generators.size() != param_index.size() || generators.size() != scale.size())
+            throw std::invalid_argument("SectorTemplate: generators, param_index and scale must have equal length");
+        std::vector<qfe_pauli> one;
+        off.assign(1, 0);
+        for (const auto& g : generators) {
+            if (!compile_pauli_sum(g, one)) throw std::invalid_argument("SectorTemplate: generators must be sums of Pauli strings");
+            terms.insert(terms.end(), one.begin(), one.end());
+            off.push_back(terms.size());
+        }
+        pi.assign(param_index.begin(), param_index.end());
+    }
+    int n_;
+    size_t nparam_;
+    size_t ninputs_ = 0;
+    qfe_template t_ = nullptr;
+};
+
+// Continuous-batching serving engine over sector templates.
 class BatchEngine {
   public:
     BatchEngine(uint64_t budget, int max_active) { check(qfe_batch_create(budget, max_active, &b_), "BatchEngine"); }
     ~BatchEngine() {
         if (b_) qfe_batch_destroy(b_);
     }
-    uint64_t submit(std::shared_ptr<UCCProgram> p, const std::vector<double>& t) {
-        if (t.size() != p->nparam()) throw std::invalid_argument("BatchEngine.submit: wrong number of amplitudes");
+    uint64_t submit(std::shared_ptr<SectorTemplate> t, const std::vector<double>& amps, const std::vector<double>& inputs) {
+        t->need(amps, inputs);
         uint64_t id = 0;
-        check(qfe_batch_submit(b_, p->handle(), t.data(), &id), "BatchEngine.submit");
-        keep_.push_back(p);
+        check(qfe_batch_submit(b_, t->handle(), amps.data(), inputs.data(), &id), "BatchEngine.submit");
+        keep_.push_back(t);
         return id;
     }
-    std::vector<std::pair<uint64_t, double>> step(size_t cap) {
+    std::vector<std::pair<uint64_t, double>> step(size_t cap) { return run(cap, true); }
+    std::vector<std::pair<uint64_t, double>> poll(size_t cap) { return run(cap, false); }
+    std::pair<uint64_t, uint64_t> pending() {
+        uint64_t q = 0, a = 0;
+        check(qfe_batch_pending(b_, &q, &a), "BatchEngine.pending");
+        return {q, a};
+    }
+
+  private:
+    std::vector<std::pair<uint64_t, double>> run(size_t cap, bool block) {
         std::vector<uint64_t> ids(cap);
         std::vector<double> es(cap);
         uint64_t nd = 0;
         {
             py::gil_scoped_release rel;
-            check(qfe_batch_step(b_, ids.data(), es.data(), cap, &nd), "BatchEngine.step");
+            if (block)
+                check(qfe_batch_step(b_, ids.data(), es.data(), cap, &nd), "BatchEngine.step");
+            else
+                check(qfe_batch_poll(b_, ids.data(), es.data(), cap, &nd), "BatchEngine.poll");
         }
         std::vector<std::pair<uint64_t, double>> out;
         for (uint64_t i = 0; i < nd; ++i) out.emplace_back(ids[i], es[i]);
@@ -148,15 +253,8 @@ class BatchEngine {
         if (q == 0 && a == 0) keep_.clear();
         return out;
     }
-    std::pair<uint64_t, uint64_t> pending() {
-        uint64_t q = 0, a = 0;
-        check(qfe_batch_pending(b_, &q, &a), "BatchEngine.pending");
-        return {q, a};
-    }
-
-  private:
     qfe_batch b_ = nullptr;
-    std::vector<std::shared_ptr<UCCProgram>> keep_;
+    std::vector<std::shared_ptr<SectorTemplate>> keep_;
 };
 
 // HIP-event timer on the engine stream.
@@ -461,10 +559,43 @@ PYBIND11_MODULE(_core, m) {
         .def_property_readonly("nqubit", &UCCProgram::nqubit)
         .def_property_readonly("nparam", &UCCProgram::nparam);
 
+    py::class_<HamiltonianMap, std::shared_ptr<HamiltonianMap>>(m, "HamiltonianMap")
+        .def(py::init([](size_t norb) { return std::make_shared<HamiltonianMap>(molecular_hamiltonian_map(norb)); }),
+             "norb"_a)
+        .def_readonly("norb", &HamiltonianMap::norb)
+        .def_readonly("ninputs", &HamiltonianMap::ninputs)
+        .def_property_readonly("nterms", [](const HamiltonianMap& mp) { return mp.x.size(); })
+        .def("pack",
+             [](const HamiltonianMap& mp, double constant, py::array_t<double, py::array::c_style | py::array::forcecast> h,
+                py::array_t<double, py::array::c_style | py::array::forcecast> g) {
+                 const size_t n = mp.norb;
+                 if ((size_t)h.size() != n * n || (size_t)g.size() != n * n * n * n)
+                     throw std::invalid_argument("HamiltonianMap.pack: h must be norb^2 and g norb^4");
+                 return pack_integrals(n, constant, h.data(), g.data());
+             },
+             "constant"_a, "h"_a, "g"_a)
+        .def("operator", &map_to_operator, "inputs"_a);
+
+    py::class_<SectorTemplate, std::shared_ptr<SectorTemplate>>(m, "SectorTemplate")
+        .def(py::init<int, uint64_t, const std::vector<QubitOperator>&, const std::vector<int>&,
+                      const std::vector<double>&, size_t, const QubitOperator&>(),
+             "nqubit"_a, "ref_index"_a, "generators"_a, "param_index"_a, "scale"_a, "nparam"_a, "hamiltonian"_a)
+        .def(py::init<int, uint64_t, const std::vector<QubitOperator>&, const std::vector<int>&,
+                      const std::vector<double>&, size_t, const HamiltonianMap&>(),
+             "nqubit"_a, "ref_index"_a, "generators"_a, "param_index"_a, "scale"_a, "nparam"_a, "hamiltonian_map"_a)
+        .def("energy", &SectorTemplate::energy, "amplitudes"_a, "inputs"_a = std::vector<double>())
+        .def("prepare", &SectorTemplate::prepare)
+        .def_property_readonly("dim", &SectorTemplate::dim)
+        .def_property_readonly("request_bytes", &SectorTemplate::request_bytes)
+        .def_property_readonly("nqubit", &SectorTemplate::nqubit)
+        .def_property_readonly("nparam", &SectorTemplate::nparam)
+        .def_property_readonly("ninputs", &SectorTemplate::ninputs);
+
     py::class_<BatchEngine>(m, "BatchEngine")
         .def(py::init<uint64_t, int>(), "hbm_budget_bytes"_a, "max_active"_a)
-        .def("submit", &BatchEngine::submit)
-        .def("step", &BatchEngine::step, "cap"_a = 1024)
+        .def("submit", &BatchEngine::submit, "template"_a, "amplitudes"_a, "inputs"_a = std::vector<double>())
+        .def("step", &BatchEngine::step, "cap"_a = 4096)
+        .def("poll", &BatchEngine::poll, "cap"_a = 4096)
         .def("pending", &BatchEngine::pending);
 
     // engine controls

@@ -305,4 +305,25 @@ void check(int rc, const char* what);
 // Compile a QubitOperator whose terms are all Pauli circuits; false if not.
 bool compile_pauli_sum(const QubitOperator& op, std::vector<qfe_pauli>& out, cplx mult = 1.0);
 
+// The JW-mapped molecular Hamiltonian of norb spatial orbitals as a linear
+// map from integrals to Pauli coefficients: term k is M(x_k, z_k) with
+// M(x, z)|j> = (-1)^{popcount(j & z)} |j ^ x>, and its coefficient is
+// sum over e in [offsets[k], offsets[k+1]) of value[e] * inputs[index[e]].
+// Inputs (ninputs = 1 + norb(norb+1)/2 + P(P+1)/2, P = norb(norb+1)/2):
+// [constant, h_pq for p >= q, (pq|rs) for pq >= rs], pair index p(p+1)/2 + q.
+// The spin-orbital expansion is that of molecule_adapters.py:183-210.
+struct HamiltonianMap {
+    size_t norb = 0;
+    size_t ninputs = 0;
+    std::vector<uint64_t> x, z;
+    std::vector<uint64_t> offsets;
+    std::vector<uint32_t> index;
+    std::vector<double> value;
+};
+HamiltonianMap molecular_hamiltonian_map(size_t norb);
+// Pack (constant, h[norb*norb], g[norb^4], chemist notation) into map inputs.
+std::vector<double> pack_integrals(size_t norb, double constant, const double* h, const double* g);
+// The Hamiltonian of a map at given inputs, as canonical Pauli circuits.
+QubitOperator map_to_operator(const HamiltonianMap& m, const std::vector<double>& inputs);
+
 }  // namespace qfa

@@ -484,4 +484,151 @@ std::string SQOpPool::str() const {
     return join(s, " ");
 }
 
+// ---- molecular Hamiltonian as a linear map of the integrals ---------------------------
+
+namespace {
+
+// A product of ladder operators in mask form: sum of c * M(x, z).
+struct MaskTerm {
+    uint64_t x, z;
+    double c;
+};
+
+// a_p^dag = 1/2 (X_p - i Y_p) Z_{<p} and a_p = 1/2 (X_p + i Y_p) Z_{<p}
+// (sq_operator.cc:106-136); with Y = i X Z as operators these are
+// 1/2 M(b, Z<p) +/- 1/2 M(b, Z<p | b), b = bit p.
+void ladder(size_t p, bool creator, MaskTerm out[2]) {
+    const uint64_t b = 1ull << p, lo = b - 1ull;
+    out[0] = {b, lo, 0.5};
+    out[1] = {b, lo | b, creator ? 0.5 : -0.5};
+}
+
+// M(x1, z1) M(x2, z2) = (-1)^{popcount(x2 & z1)} M(x1 ^ x2, z1 ^ z2)
+std::vector<MaskTerm> ladder_product(const std::vector<size_t>& cre, const std::vector<size_t>& ann) {
+    std::vector<MaskTerm> prod{{0, 0, 1.0}}, next;
+    auto mul = [&](size_t p, bool creator) {
+        MaskTerm f[2];
+        ladder(p, creator, f);
+        next.clear();
+        for (const auto& a : prod)
+            for (const auto& b : f) {
+                const double sg = (__builtin_popcountll(b.x & a.z) & 1) ? -1.0 : 1.0;
+                next.push_back({a.x ^ b.x, a.z ^ b.z, sg * a.c * b.c});
+            }
+        // combine equal masks (all coefficients are dyadic: the sums are exact)
+        std::map<std::pair<uint64_t, uint64_t>, double> acc;
+        for (const auto& t : next) acc[{t.x, t.z}] += t.c;
+        prod.clear();
+        for (const auto& kv : acc)
+            if (kv.second != 0.0) prod.push_back({kv.first.first, kv.first.second, kv.second});
+    };
+    for (size_t p : cre) mul(p, true);
+    for (size_t p : ann) mul(p, false);
+    return prod;
+}
+
+inline size_t pair_index(size_t p, size_t q) { return p >= q ? p * (p + 1) / 2 + q : q * (q + 1) / 2 + p; }
+
+}  // namespace
+
+HamiltonianMap molecular_hamiltonian_map(size_t norb) {
+    if (norb == 0 || norb > 32) throw std::invalid_argument("molecular_hamiltonian_map: norb must be in [1, 32]");
+    const size_t P = norb * (norb + 1) / 2;
+    const size_t h0 = 1, g0 = 1 + P;
+    auto gidx = [&](size_t p, size_t q, size_t r, size_t s) { return g0 + pair_index(pair_index(p, q), pair_index(r, s)); };
+    std::map<std::pair<uint64_t, uint64_t>, std::map<uint32_t, double>> acc;
+    std::vector<std::pair<uint64_t, uint64_t>> order;
+    auto add = [&](double factor, size_t input, const std::vector<size_t>& cre, const std::vector<size_t>& ann) {
+        for (const auto& t : ladder_product(cre, ann)) {
+            auto key = std::make_pair(t.x, t.z);
+            auto it = acc.find(key);
+            if (it == acc.end()) {
+                it = acc.emplace(key, std::map<uint32_t, double>()).first;
+                order.push_back(key);
+            }
+            it->second[(uint32_t)input] += factor * t.c;
+        }
+    };
+    add(1.0, 0, {}, {});
+    // the loop structure and spin cases of molecule_adapters.py:183-210
+    for (size_t i = 0; i < norb; ++i) {
+        const size_t ia = 2 * i, ib = 2 * i + 1;
+        for (size_t j = 0; j < norb; ++j) {
+            const size_t ja = 2 * j, jb = 2 * j + 1;
+            add(1.0, h0 + pair_index(i, j), {ia}, {ja});
+            add(1.0, h0 + pair_index(i, j), {ib}, {jb});
+            for (size_t k = 0; k < norb; ++k) {
+                const size_t ka = 2 * k, kb = 2 * k + 1;
+                for (size_t l = 0; l < norb; ++l) {
+                    const size_t la = 2 * l, lb = 2 * l + 1;
+                    const size_t g = gidx(i, l, k, j);  // (il|kj) / 2
+                    if (ia != jb && kb != la) add(0.5, g, {ia, jb}, {kb, la});
+                    if (ib != ja && ka != lb) add(0.5, g, {ib, ja}, {ka, lb});
+                    if (ia != ja && ka != la) add(0.5, g, {ia, ja}, {ka, la});
+                    if (ib != jb && kb != lb) add(0.5, g, {ib, jb}, {kb, lb});
+                }
+            }
+        }
+    }
+    HamiltonianMap m;
+    m.norb = norb;
+    m.ninputs = g0 + P * (P + 1) / 2;
+    m.offsets.push_back(0);
+    for (const auto& key : order) {
+        const auto& row = acc[key];
+        size_t kept = 0;
+        for (const auto& kv : row)
+            if (std::fabs(kv.second) > 1e-15) {
+                m.index.push_back(kv.first);
+                m.value.push_back(kv.second);
+                ++kept;
+            }
+        if (!kept) continue;
+        m.x.push_back(key.first);
+        m.z.push_back(key.second);
+        m.offsets.push_back(m.index.size());
+    }
+    return m;
+}
+
+std::vector<double> pack_integrals(size_t norb, double constant, const double* h, const double* g) {
+    const size_t P = norb * (norb + 1) / 2;
+    std::vector<double> in(1 + P + P * (P + 1) / 2, 0.0);
+    in[0] = constant;
+    for (size_t p = 0; p < norb; ++p)
+        for (size_t q = 0; q <= p; ++q) in[1 + pair_index(p, q)] = h[p * norb + q];
+    for (size_t p = 0; p < norb; ++p)
+        for (size_t q = 0; q <= p; ++q)
+            for (size_t r = 0; r < norb; ++r)
+                for (size_t s = 0; s <= r; ++s) {
+                    const size_t pq = pair_index(p, q), rs = pair_index(r, s);
+                    if (pq < rs) continue;
+                    in[1 + P + pair_index(pq, rs)] = g[((p * norb + q) * norb + r) * norb + s];
+                }
+    return in;
+}
+
+QubitOperator map_to_operator(const HamiltonianMap& m, const std::vector<double>& inputs) {
+    if (inputs.size() != m.ninputs) throw std::invalid_argument("map_to_operator: wrong number of inputs");
+    QubitOperator op;
+    const cplx mi(0.0, -1.0);
+    for (size_t k = 0; k < m.x.size(); ++k) {
+        double c = 0.0;
+        for (uint64_t e = m.offsets[k]; e < m.offsets[k + 1]; ++e) c += m.value[e] * inputs[m.index[e]];
+        // canonical circuit C(x, z) = i^{popcount(x & z)} M(x, z)
+        cplx coef = c;
+        for (int y = __builtin_popcountll(m.x[k] & m.z[k]); y > 0; --y) coef *= mi;
+        Circuit circ;
+        for (size_t q = 0; q < 64; ++q) {
+            const bool bx = (m.x[k] >> q) & 1ull, bz = (m.z[k] >> q) & 1ull;
+            if (bx && bz) circ.add_gate(make_gate("Y", q, q));
+            else if (bx) circ.add_gate(make_gate("X", q, q));
+            else if (bz) circ.add_gate(make_gate("Z", q, q));
+        }
+        op.add_term(coef, circ);
+    }
+    return op;
+}
+
+
 }  // namespace qfa

@@ -50,3 +50,12 @@ def compile_ucc_program(nqubit, reference, hamiltonian, pool, tops, pool_type=No
     tops[0] first) on the determinant ``reference`` (occupation list)."""
     gens, index, scale = ucc_generators(pool, tops, pool_type, qubit_excitations)
     return _core.UCCProgram(nqubit, ref_to_basis_idx(reference), gens, index, scale, len(tops), hamiltonian)
+
+
+def compile_ucc_template(nqubit, reference, hamiltonian, pool, tops, pool_type=None, qubit_excitations=False):
+    """A ``SectorTemplate``: the same ansatz as ``compile_ucc_program`` kept on
+    the determinants of the reference's (N_alpha, N_beta) sector.
+    ``hamiltonian`` is a ``QubitOperator`` (fixed) or a ``HamiltonianMap``
+    (coefficients from per-request integrals)."""
+    gens, index, scale = ucc_generators(pool, tops, pool_type, qubit_excitations)
+    return _core.SectorTemplate(nqubit, ref_to_basis_idx(reference), gens, index, scale, len(tops), hamiltonian)

@@ -1,36 +1,49 @@
 """Continuous-batching serving loop for UCC energy requests.
 
-Requests carry a compiled program and its amplitudes.  The native engine
-(``qfe_batch``) admits them into active slots while the HBM budget
-(16 * 2^n bytes per state) allows, advances all active requests together and
-retires them as they finish; this module adds arrival-time bookkeeping and
-latency statistics.
+A request is a ``SectorTemplate`` (one size class: ansatz and Hamiltonian
+structure), its amplitudes and its Hamiltonian inputs (integrals packed by
+``HamiltonianMap.pack``).  The native engine (``qfe_batch``) queues requests
+on the host; every ``step``/``poll`` launches what can be admitted onto one of
+a few streams (one workgroup per request, requests of any size class in the
+same launch, bounded by the HBM budget), so work that arrives while other
+launches run is admitted into the next launch and launches overlap on the
+device.  This module adds arrival-time bookkeeping and latency statistics.
 """
 
 import time
+
+import numpy as np
 
 from .. import _core
 
 
 class ServingEngine:
-    def __init__(self, hbm_budget_bytes=64 << 30, max_active=256):
+    def __init__(self, hbm_budget_bytes=64 << 30, max_active=4096):
         self._eng = _core.BatchEngine(int(hbm_budget_bytes), int(max_active))
         self._arrival = {}
         self.latencies = {}
         self.results = {}
 
-    def submit(self, program, amplitudes, arrival_time=None):
-        rid = self._eng.submit(program, list(amplitudes))
+    def submit(self, template, amplitudes, inputs=(), arrival_time=None):
+        rid = self._eng.submit(template, list(amplitudes), list(inputs))
         self._arrival[rid] = time.perf_counter() if arrival_time is None else arrival_time
         return rid
 
-    def step(self):
-        done = self._eng.step()
+    def _record(self, done):
         now = time.perf_counter()
         for rid, e in done:
             self.results[rid] = e
             self.latencies[rid] = now - self._arrival.pop(rid)
         return done
+
+    def step(self):
+        """Launch what can be admitted; wait for the oldest launch only when
+        nothing has completed yet."""
+        return self._record(self._eng.step())
+
+    def poll(self):
+        """Launch what can be admitted and return what has completed, without waiting."""
+        return self._record(self._eng.poll())
 
     def pending(self):
         return self._eng.pending()
@@ -41,3 +54,7 @@ class ServingEngine:
             if q == 0 and a == 0:
                 return
             self.step()
+
+    def latency_percentiles(self, ids=None, q=(50, 99)):
+        lat = np.array([self.latencies[i] for i in (ids if ids is not None else self.latencies)])
+        return {f"p{p}": float(np.percentile(lat, p)) for p in q}

@@ -1,0 +1,140 @@
+"""Determinant-sector templates and the serving engine.
+
+The sector path must reproduce the full-space compiled program (itself
+pinned against the dense oracle and the reference's gate-level path): the
+prepared state amplitude by amplitude (1e-12) and the energy (1e-10 Ha), for
+fixed Hamiltonians and for Hamiltonians built from per-request integrals by
+``HamiltonianMap``.  The serving engine must return, for every request, the
+same energy in a mixed batch as alone."""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+E_TOL = 1e-10
+
+
+def sd_pool(qf, ref):
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(ref)
+    pool.fill_pool("SD")
+    return pool
+
+
+def synthetic_case(qf, norb, nalpha, nbeta, seed):
+    h, g = qf.synthetic_integrals(norb, seed)
+    ref = qf.system.hamiltonians.hf_occupation(norb, nalpha, nbeta)
+    return h, g, ref
+
+
+def test_h4_template_matches_program(qf):
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    n = len(mol.hf_reference)
+    for pool_type in ("SD", "GSD", "SDTQ"):
+        pool = qf.SQOpPool()
+        pool.set_orb_spaces(mol.hf_reference)
+        pool.fill_pool(pool_type)
+        tops = list(range(len(pool)))
+        prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+        tpl = qf.compile_ucc_template(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+        assert tpl.dim == 36  # C(4,2)^2 determinants of 2 alpha + 2 beta electrons in 4 orbitals
+        for seed in (1, 2):
+            t = list(np.random.default_rng(seed).normal(0.0, 0.2, len(tops)))
+            assert abs(tpl.energy(t) - prog.energy(t)) < E_TOL, pool_type
+            a, b = qf.Computer(n), qf.Computer(n)
+            prog.prepare(t, a)
+            tpl.prepare(t, b)
+            diff = np.abs(np.asarray(a.get_coeff_vec()) - np.asarray(b.get_coeff_vec())).max()
+            assert diff < 1e-12, (pool_type, diff)
+
+
+def test_template_energy_against_dense_oracle(qf, oracle):
+    from test_gpu_ucc import oracle_ucc_energy, terms_list
+
+    h, g, ref = synthetic_case(qf, 4, 2, 2, 11)
+    H = qf.build_sq_hamiltonian(h, g, 0.25).jw_transform()
+    pool = sd_pool(qf, ref)
+    tops = list(range(len(pool)))
+    tpl = qf.compile_ucc_template(8, ref, H, pool, tops)
+    gens, _, _ = qf.ucc_generators(pool, tops)
+    t = list(np.random.default_rng(3).normal(0.0, 0.05, len(tops)))
+    e_or, _ = oracle_ucc_energy(oracle, 8, ref, gens, t, terms_list(H))
+    assert abs(tpl.energy(t) - e_or) < E_TOL
+
+
+@pytest.mark.parametrize("norb,na,nb", [(4, 2, 2), (5, 3, 2), (6, 3, 3), (7, 4, 3)])
+def test_integral_map_template_matches_program(qf, norb, na, nb):
+    n = 2 * norb
+    h, g, ref = synthetic_case(qf, norb, na, nb, 100 + norb)
+    H = qf.build_sq_hamiltonian(h, g, 0.5).jw_transform()
+    pool = sd_pool(qf, ref)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, ref, H, pool, tops)
+    hmap = qf.HamiltonianMap(norb)
+    tpl = qf.compile_ucc_template(n, ref, hmap, pool, tops)
+    inputs = hmap.pack(0.5, h, g)
+    t = list(np.random.default_rng(norb).normal(0.0, 0.05, len(tops)))
+    assert abs(tpl.energy(t, inputs) - prog.energy(t)) < E_TOL
+
+
+def test_non_conserving_generator_is_rejected(qf):
+    # a single Pauli string changes the particle number: no sector form exists
+    gen = qf.QubitOperator()
+    gen.add(0.5j, qf.build_circuit("X_0 Y_1"))
+    H = qf.QubitOperator()
+    H.add(1.0, qf.build_circuit("Z_0"))
+    with pytest.raises(Exception, match="conserve|anti-Hermitian"):
+        qf.SectorTemplate(4, 0b0011, [gen], [0], [1.0], 1, H)
+
+
+def test_serving_results_are_the_same_alone_and_in_the_mix(qf):
+    cases = []
+    for norb, na, nb in [(4, 2, 2), (5, 3, 2), (6, 3, 3), (7, 4, 3), (8, 4, 4)]:
+        _, _, ref = synthetic_case(qf, norb, na, nb, 0)
+        pool = sd_pool(qf, ref)
+        hmap = qf.HamiltonianMap(norb)
+        tpl = qf.compile_ucc_template(2 * norb, ref, hmap, pool, list(range(len(pool))))
+        cases.append((norb, tpl, hmap))
+    rng = np.random.default_rng(5)
+    reqs = []
+    for k in range(60):
+        norb, tpl, hmap = cases[k % len(cases)]
+        h, g = qf.synthetic_integrals(norb, 1000 + k)
+        inputs = hmap.pack(0.0, h, g)
+        t = list(rng.normal(0.0, 0.05, tpl.nparam))
+        reqs.append((tpl, t, inputs))
+    alone = [tpl.energy(t, inp) for tpl, t, inp in reqs]
+    eng = qf.ServingEngine(hbm_budget_bytes=1 << 30, max_active=4096)
+    ids = []
+    # arrivals in three waves: later waves are admitted while earlier launches run
+    for wave in range(3):
+        for tpl, t, inp in reqs[wave * 20:(wave + 1) * 20]:
+            ids.append(eng.submit(tpl, t, inp))
+        eng.poll()
+    eng.drain()
+    mixed = [eng.results[i] for i in ids]
+    assert np.max(np.abs(np.array(mixed) - np.array(alone))) < 1e-12
+    p = eng.latency_percentiles()
+    assert 0 < p["p50"] <= p["p99"]
+
+
+def test_serving_respects_the_budget(qf):
+    _, _, ref = synthetic_case(qf, 6, 3, 3, 0)
+    pool = sd_pool(qf, ref)
+    hmap = qf.HamiltonianMap(6)
+    tpl = qf.compile_ucc_template(12, ref, hmap, pool, list(range(len(pool))))
+    h, g = qf.synthetic_integrals(6, 1)
+    inputs = hmap.pack(0.0, h, g)
+    t = [0.01] * tpl.nparam
+    # room for about two requests per launch: every request still completes
+    eng = qf.ServingEngine(hbm_budget_bytes=3 * 2 * tpl.request_bytes + 1024, max_active=8)
+    ids = [eng.submit(tpl, t, inputs) for _ in range(10)]
+    eng.drain()
+    e = tpl.energy(t, inputs)
+    assert all(abs(eng.results[i] - e) < 1e-12 for i in ids)
+    with pytest.raises(Exception, match="budget"):
+        small = qf.ServingEngine(hbm_budget_bytes=tpl.request_bytes, max_active=8)
+        small.submit(tpl, t, inputs)

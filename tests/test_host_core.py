@@ -243,3 +243,24 @@ def test_exponentiate_gate_list_matches_construction(qf, ref, oracle):
         assert [g.gate_id() for g in circ.gates()] == [g.gtype for g in expect]
         assert [(g.target(), g.control()) for g in circ.gates()] == [(g.target, g.control) for g in expect]
         assert circ.num_exponential_segments() == 1 and ph == 1.0
+
+
+@pytest.mark.parametrize("norb", [2, 3, 4, 6])
+def test_hamiltonian_map_equals_jw_of_the_integral_hamiltonian(qf, norb):
+    # molecule_adapters.py:183-210 then jw_transform, against the linear map
+    # from packed integrals to Pauli coefficients: same terms, same values
+    h, g = qf.synthetic_integrals(norb, 7)
+    hmap = qf.HamiltonianMap(norb)
+    op = hmap.operator(hmap.pack(0.37, h, g))
+    ref = qf.build_sq_hamiltonian(h, g, 0.37).jw_transform()
+
+    def as_dict(o):
+        d = {}
+        for c, circ in o.terms():
+            d[str(circ)] = d.get(str(circ), 0) + c
+        return d
+
+    a, b = as_dict(op), as_dict(ref)
+    assert set(a) == set(b)
+    assert max(abs(a[k] - b[k]) for k in a) < 1e-14
+    assert hmap.nterms == len(ref.terms())
