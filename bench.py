@@ -84,6 +84,115 @@ def time_single_sweep(qf, W, qc, n, reps=40):
     return ms / reps
 
 
+def run_open_loop(qf, classes, reqs, rate, seed):
+    """Poisson arrivals at `rate` requests/s into a fresh serving engine;
+    returns (makespan seconds, latencies in arrival order, results)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    eng = qf.ServingEngine(hbm_budget_bytes=64 << 30, max_active=8192)
+    ids = []
+    i = 0
+    t0 = time.perf_counter()
+    while len(eng.results) < len(reqs):
+        now = time.perf_counter() - t0
+        while i < len(reqs) and arrivals[i] <= now:
+            n, inputs, t = reqs[i]
+            ids.append(eng.submit(classes[n]["template"], t, inputs, arrival_time=t0 + arrivals[i]))
+            i += 1
+        if not eng.poll() and i < len(reqs):
+            wait = arrivals[i] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(min(wait, 2e-4))
+    makespan = time.perf_counter() - t0
+    lat = np.array([eng.latencies[r] for r in ids])
+    return makespan, lat, [eng.results[r] for r in ids]
+
+
+def serving_leg(qf, W, nreq, plain_per_class):
+    """Continuous-batching UCC energy serving: capacity (all requests queued
+    at once), three Poisson rates relative to it, and the plain
+    one-request-per-launch paths on a per-class sample."""
+    import numpy as np
+
+    t_setup = time.perf_counter()
+    classes = W.ucc_size_classes(qf)
+    reqs = W.ucc_requests(qf, classes, nreq, seed=7)
+    setup_s = time.perf_counter() - t_setup
+    print(f"[bench] serving: {len(classes)} size classes compiled, {nreq} requests built in {setup_s:.1f} s",
+          file=sys.stderr, flush=True)
+
+    # warm every class once (first-launch costs), then capacity: all queued at t=0
+    for n, c in classes.items():
+        c["template"].energy([0.0] * c["template"].nparam, [0.0] * c["template"].ninputs)
+    eng = qf.ServingEngine(hbm_budget_bytes=64 << 30, max_active=8192)
+    t0 = time.perf_counter()
+    ids = [eng.submit(classes[n]["template"], t, inputs, arrival_time=t0) for n, inputs, t in reqs]
+    eng.drain()
+    cap_s = time.perf_counter() - t0
+    batched = [eng.results[r] for r in ids]
+    capacity = nreq / cap_s
+    print(f"[bench] serving capacity {capacity:.0f} evals/s ({cap_s:.3f} s for {nreq})", file=sys.stderr, flush=True)
+
+    rates = {}
+    for f in (0.5, 0.9, 1.5):
+        span, lat, res = run_open_loop(qf, classes, reqs, f * capacity, seed=int(f * 10))
+        same = float(np.max(np.abs(np.array(res) - np.array(batched))))
+        rates[f"{f}x"] = {
+            "offered_per_s": round(f * capacity, 1),
+            "evals_per_s": round(nreq / span, 1),
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "max_abs_diff_vs_capacity_run": same,
+        }
+        print(f"[bench] serving at {f}x: {rates[f'{f}x']}", file=sys.stderr, flush=True)
+
+    # plain paths on a sample: per class, the first plain_per_class requests
+    by_class = {n: [i for i, r in enumerate(reqs) if r[0] == n][:plain_per_class] for n in classes}
+    plain_full, plain_sector, create_full, max_err = {}, {}, {}, 0.0
+    for n, idx in by_class.items():
+        c = classes[n]
+        gens, pidx, scale = qf.ucc_generators(c["pool"], c["tops"])
+        tf, ts, tc = [], [], []
+        for i in idx:
+            _, inputs, t = reqs[i]
+            a = time.perf_counter()
+            prog = qf.UCCProgram(n, qf.ref_to_basis_idx(c["ref"]), gens, pidx, scale, len(c["tops"]),
+                                 c["map"].operator(inputs))
+            b = time.perf_counter()
+            e_full = prog.energy(t)
+            d = time.perf_counter()
+            e_sec = c["template"].energy(t, inputs)
+            f = time.perf_counter()
+            tc.append(b - a)
+            tf.append(d - b)
+            ts.append(f - d)
+            max_err = max(max_err, abs(e_full - batched[i]), abs(e_sec - batched[i]))
+            del prog
+        plain_full[n], plain_sector[n], create_full[n] = float(np.mean(tf)), float(np.mean(ts)), float(np.mean(tc))
+        print(f"[bench] plain n={n}: full-space {1e3 * plain_full[n]:.3f} ms/eval (+{1e3 * create_full[n]:.1f} ms "
+              f"program build), sector one-launch {1e3 * plain_sector[n]:.3f} ms/eval", file=sys.stderr, flush=True)
+    # the request mix is uniform over the classes
+    plain_rate = 1.0 / float(np.mean(list(plain_full.values())))
+    sector_rate = 1.0 / float(np.mean(list(plain_sector.values())))
+    return {
+        "workload": (f"{nreq} UCCSD energy requests, n uniform over {sorted(classes)}, half filled, own seeded "
+                     "synthetic integrals per request, amplitudes N(0,0.05)"),
+        "capacity_evals_per_s": round(capacity, 1),
+        "poisson": rates,
+        "plain_full_space_evals_per_s": round(plain_rate, 2),
+        "plain_sector_one_launch_evals_per_s": round(sector_rate, 2),
+        "speedup_vs_plain": round(capacity / plain_rate, 1),
+        "speedup_vs_plain_sector": round(capacity / sector_rate, 1),
+        "plain_ms_per_eval_by_n": {n: round(1e3 * v, 3) for n, v in plain_full.items()},
+        "plain_program_build_ms_by_n": {n: round(1e3 * v, 2) for n, v in create_full.items()},
+        "plain_sample_per_class": plain_per_class,
+        "max_abs_energy_diff_vs_plain": max_err,
+        "sector_dims": {n: c["template"].dim for n, c in classes.items()},
+    }
+
+
 def cpu_baseline(W, n_cpu, oracle_ref_loader):
     """Reference Computer (oracle/_ref), one thread, on a bounded sample:
     1 brickwork layer + 5 exponentials + 5 expectation terms at n_cpu qubits."""
@@ -138,6 +247,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true", help="skip the dominant-kernel timing")
     ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops")
+    ap.add_argument("--no-serving", action="store_true", help="skip the UCC serving leg")
+    ap.add_argument("--serving-only", action="store_true", help="only the UCC serving leg (printed as JSON)")
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--plain-per-class", type=int, default=8)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the exponential stage and a calibration sweep (for hardware-counter runs)")
     args = ap.parse_args()
@@ -157,6 +270,10 @@ def main():
         sys.exit(1)
     if args.tuning:
         qf.set_tuning(*[int(v) for v in args.tuning.split(",")])
+
+    if args.serving_only:
+        print(json.dumps(serving_leg(qf, W, args.requests, args.plain_per_class)))
+        return
 
     n = args.nqubit
     print(f"[bench] building workload n={n} layers={args.layers} exps={args.nexp} terms={args.nterms}",
@@ -241,6 +358,10 @@ def main():
                 "seconds": round(res["seconds"], 3),
                 "gpu_effective_GBps_same_unit": round(eff_gbps, 1),
             }
+    serving = None
+    if not args.no_serving:
+        del qc
+        serving = serving_leg(qf, W, args.requests, args.plain_per_class)
     line = {
         "metric": METRIC,
         "value": round(gbps, 1),
@@ -272,6 +393,7 @@ def main():
         "norm_error": norm_err,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line))
 

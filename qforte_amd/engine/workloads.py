@@ -111,3 +111,55 @@ def exp_operator(mod, strings):
     for theta, term in strings:
         op.add(-1j * theta, term_circuit(mod, term))
     return op
+
+
+# ---- UCC serving workload ---------------------------------------------------------------
+
+
+def half_filled_reference(nqubit):
+    """Occupation list of the half-filled determinant of ``nqubit`` spin
+    orbitals: nqubit/2 electrons, N_alpha = ceil(N/2), N_beta = floor(N/2),
+    alpha on even and beta on odd spin orbitals (state_prep.py:5-11)."""
+    norb = nqubit // 2
+    ne = nqubit // 2
+    na, nb = (ne + 1) // 2, ne // 2
+    ref = [0] * nqubit
+    for a in range(na):
+        ref[2 * a] = 1
+    for b in range(nb):
+        ref[2 * b + 1] = 1
+    return norb, na, nb, ref
+
+
+def ucc_size_classes(mod, sizes=(8, 10, 12, 14, 16, 18, 20)):
+    """One compiled UCCSD size class per qubit count: the SD pool of the
+    half-filled reference, the integral map, and the sector template."""
+    out = {}
+    for n in sizes:
+        norb, na, nb, ref = half_filled_reference(n)
+        pool = mod.SQOpPool()
+        pool.set_orb_spaces(ref)
+        pool.fill_pool("SD")
+        tops = list(range(len(pool)))
+        hmap = mod.HamiltonianMap(norb)
+        tpl = mod.compile_ucc_template(n, ref, hmap, pool, tops)
+        out[n] = {"norb": norb, "ref": ref, "pool": pool, "tops": tops, "map": hmap, "template": tpl}
+    return out
+
+
+def ucc_requests(mod, classes, count, seed):
+    """``count`` requests with qubit counts uniform over the classes, each
+    with its own seeded synthetic integrals (h symmetric N(0, 0.1) - 2p/norb on
+    the diagonal, g 8-fold symmetric N(0, 0.01) + 0.5 on g_pppp) and amplitudes
+    N(0, 0.05).  Returns (n, inputs, amplitudes) tuples."""
+    rng = np.random.default_rng(seed)
+    sizes = sorted(classes)
+    out = []
+    for k in range(count):
+        n = sizes[int(rng.integers(len(sizes)))]
+        c = classes[n]
+        h, g = mod.synthetic_integrals(c["norb"], seed * 1000003 + k)
+        inputs = c["map"].pack(0.0, h, g)
+        t = rng.normal(0.0, 0.05, c["template"].nparam)
+        out.append((n, inputs, [float(v) for v in t]))
+    return out

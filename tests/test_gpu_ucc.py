@@ -181,3 +181,30 @@ def test_sa_sd_generators_share_amplitudes(qf, oracle, h4):
         tp[k] += h
         tm[k] -= h
         assert abs((prog.energy(list(tp)) - prog.energy(list(tm))) / (2 * h) - g[k]) < 1e-8
+
+
+def test_autograd_function_backward_is_the_adjoint_gradient(qf, h4):
+    # torch.autograd.Function over the engine: forward = energy, backward =
+    # the adjoint gradient, checked against central differences (h = 1e-5)
+    import torch
+
+    from qforte_amd.engine.autograd import UCCEnergy
+
+    mol, pool = h4
+    n = len(mol.hf_reference)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    t = torch.tensor(np.random.default_rng(9).normal(0.0, 0.05, len(tops)), dtype=torch.float64, requires_grad=True)
+    e = UCCEnergy.apply(t, prog)
+    assert abs(e.item() - prog.energy(t.detach().numpy().tolist())) < 1e-14
+    (2.5 * e).backward()
+    _, g = prog.gradient(t.detach().numpy().tolist())
+    assert np.max(np.abs(t.grad.numpy() - 2.5 * np.array(g))) < 1e-14
+    h = 1e-5
+    base = t.detach().numpy()
+    for k in (0, len(tops) // 2, len(tops) - 1):
+        tp, tm = base.copy(), base.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (prog.energy(list(tp)) - prog.energy(list(tm))) / (2 * h)
+        assert abs(2.5 * fd - t.grad[k].item()) < 2.5e-8
