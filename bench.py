@@ -193,6 +193,105 @@ def serving_leg(qf, W, nreq, plain_per_class):
     }
 
 
+def timed(fn, reps=3):
+    """Median wall time (s) of fn() over reps calls (each call returns to the host)."""
+    ts, out = [], None
+    for _ in range(reps):
+        a = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - a)
+    return sorted(ts)[len(ts) // 2], out
+
+
+def gradient_leg(qf, W, nqubit=20, seed=20, fd_components=6, full_pool_sample=128):
+    """Full UCCSD gradient at 20 qubits through torch.autograd (adjoint
+    backward), checked against central differences and the full-space
+    program, plus ADAPT GSD pool gradients in batches of 64."""
+    import numpy as np
+    import torch
+
+    from qforte_amd.engine.autograd import SectorProgram, UCCEnergy
+
+    norb, na, nb, ref = W.half_filled_reference(nqubit)
+    h, g = qf.synthetic_integrals(norb, seed)
+    hmap = qf.HamiltonianMap(norb)
+    inputs = hmap.pack(0.0, h, g)
+    H = hmap.operator(inputs)
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(ref)
+    pool.fill_pool("SD")
+    tops = list(range(len(pool)))
+    t = np.random.default_rng(seed).normal(0.0, 0.05, len(tops))
+    tl = [float(v) for v in t]
+
+    a = time.perf_counter()
+    prog = qf.compile_ucc_program(nqubit, ref, H, pool, tops)
+    build_full = time.perf_counter() - a
+    a = time.perf_counter()
+    tpl = qf.compile_ucc_template(nqubit, ref, hmap, pool, tops)
+    build_sector = time.perf_counter() - a
+    sp = SectorProgram(tpl, inputs)
+
+    prog.energy(tl)
+    sp.gradient(tl)  # warm
+    t_e_full, e_full = timed(lambda: prog.energy(tl))
+    t_g_full, (_, g_full) = timed(lambda: prog.gradient(tl))
+    t_e_sec, e_sec = timed(lambda: sp.energy(tl))
+
+    def autograd_step():
+        x = torch.tensor(t, dtype=torch.float64, requires_grad=True)
+        UCCEnergy.apply(x, sp).backward()
+        return x.grad.numpy().copy()
+
+    t_g_sec, g_auto = timed(autograd_step)
+    g_full = np.array(g_full)
+    fd_err = 0.0
+    hstep = 1e-5
+    for k in np.random.default_rng(1).choice(len(tops), fd_components, replace=False):
+        tp, tm = t.copy(), t.copy()
+        tp[k] += hstep
+        tm[k] -= hstep
+        fd = (sp.energy(list(tp)) - sp.energy(list(tm))) / (2 * hstep)
+        fd_err = max(fd_err, abs(fd - g_auto[k]))
+    print(f"[bench] gradient n={nqubit}: full-space {1e3 * t_g_full:.1f} ms, sector autograd {1e3 * t_g_sec:.1f} ms",
+          file=sys.stderr, flush=True)
+
+    gsd = qf.SQOpPool()
+    gsd.set_orb_spaces(ref)
+    gsd.fill_pool("GSD")
+    a = time.perf_counter()
+    ops, _, _ = qf.ucc_generators(gsd, list(range(len(gsd))))
+    jw_s = time.perf_counter() - a
+    t_pool_sec, pg = timed(lambda: tpl.pool_gradients(tl, ops, 64, inputs), reps=2)
+    sample = ops[:full_pool_sample]
+    t_pool_full, pg_full = timed(lambda: prog.pool_gradients(tl, sample, 64), reps=1)
+    pool_err = float(np.max(np.abs(np.array(pg[: len(sample)]) - np.array(pg_full))))
+    print(f"[bench] ADAPT GSD pool: {len(ops)} ops, sector {1e3 * t_pool_sec:.1f} ms, full-space "
+          f"{1e3 * t_pool_full / len(sample):.3f} ms/op", file=sys.stderr, flush=True)
+    return {
+        "workload": (f"UCCSD gradient at {nqubit} qubits ({norb} orbitals, {na}a+{nb}b, {len(tops)} amplitudes, "
+                     f"{sum(len(q.terms()) for q in qf.ucc_generators(pool, tops)[0])} Pauli exponentials, "
+                     f"{hmap.nterms} Hamiltonian terms), synthetic integrals seed {seed}, amplitudes N(0,0.05)"),
+        "sector_dim": tpl.dim,
+        "energy_ms": {"full_space": round(1e3 * t_e_full, 3), "sector": round(1e3 * t_e_sec, 3)},
+        "gradient_ms": {"full_space_adjoint": round(1e3 * t_g_full, 3), "sector_adjoint_autograd": round(1e3 * t_g_sec, 3)},
+        "build_ms": {"full_space": round(1e3 * build_full, 1), "sector": round(1e3 * build_sector, 1)},
+        "max_abs_grad_diff_sector_vs_full": float(np.max(np.abs(g_auto - g_full))),
+        "energy_diff_sector_vs_full": abs(e_sec - e_full),
+        "max_abs_fd_error": fd_err,
+        "fd_step": hstep,
+        "adapt_gsd_pool": {
+            "ops": len(ops),
+            "batch": 64,
+            "sector_ms": round(1e3 * t_pool_sec, 3),
+            "full_space_ms_per_op": round(1e3 * t_pool_full / len(sample), 4),
+            "full_space_sample": len(sample),
+            "max_abs_diff_vs_full_space": pool_err,
+            "jw_compile_s": round(jw_s, 2),
+        },
+    }
+
+
 def cpu_baseline(W, n_cpu, oracle_ref_loader):
     """Reference Computer (oracle/_ref), one thread, on a bounded sample:
     1 brickwork layer + 5 exponentials + 5 expectation terms at n_cpu qubits."""
@@ -248,6 +347,8 @@ def main():
     ap.add_argument("--no-roofline", action="store_true", help="skip the dominant-kernel timing")
     ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops")
     ap.add_argument("--no-serving", action="store_true", help="skip the UCC serving leg")
+    ap.add_argument("--no-gradient", action="store_true", help="skip the 20-qubit gradient leg")
+    ap.add_argument("--gradient-only", action="store_true", help="only the 20-qubit gradient leg (printed as JSON)")
     ap.add_argument("--serving-only", action="store_true", help="only the UCC serving leg (printed as JSON)")
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--plain-per-class", type=int, default=8)
@@ -273,6 +374,9 @@ def main():
 
     if args.serving_only:
         print(json.dumps(serving_leg(qf, W, args.requests, args.plain_per_class)))
+        return
+    if args.gradient_only:
+        print(json.dumps(gradient_leg(qf, W)))
         return
 
     n = args.nqubit
@@ -362,6 +466,9 @@ def main():
     if not args.no_serving:
         del qc
         serving = serving_leg(qf, W, args.requests, args.plain_per_class)
+    gradient = None
+    if not args.no_gradient:
+        gradient = gradient_leg(qf, W)
     line = {
         "metric": METRIC,
         "value": round(gbps, 1),
@@ -394,6 +501,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "gradient": gradient,
     }
     print(json.dumps(line))
 

@@ -219,6 +219,16 @@ int qfe_template_info(qfe_template t, uint64_t* sector_dim, uint64_t* request_by
 int qfe_template_energy(qfe_template t, const double* params, const double* inputs, double* energy);
 /* U|ref> scattered into a full 2^n state (for checks against the full-space path). */
 int qfe_template_prepare(qfe_template t, const double* params, qfe_state out);
+/* Adjoint gradient dE/dt on the sector (uccvqeabc.py:122-251 recursion):
+ * sigma = H psi, then a reverse sweep un-applying each generator on psi and
+ * sigma; grad[p] = sum over generators k with param p of 2 scale_k
+ * Re<sigma_k|K_k|psi_k>.  energy (may be NULL) receives <psi|H|psi>. */
+int qfe_template_gradient(qfe_template t, const double* params, const double* inputs, double* grad, double* energy);
+/* ADAPT pool gradients 2 Re<H psi|A_k psi> (uccvqeabc.py:253-283) for nops
+ * anti-Hermitian operators given as Pauli sums (pool_terms split by
+ * op_offsets); `batch` operators per launch (0: 64). */
+int qfe_template_pool_gradients(qfe_template t, const double* params, const double* inputs, const qfe_pauli* pool_terms,
+                                const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 
 /* ---- serving: continuous batching of UCC energy requests ----------------
  * No reference counterpart: the reference evaluates one energy per Python

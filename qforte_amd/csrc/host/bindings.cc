@@ -171,6 +171,31 @@ class SectorTemplate {
         py::gil_scoped_release rel;
         check(qfe_template_prepare(t_, t.data(), out.handle()), "SectorTemplate.prepare");
     }
+    std::pair<double, std::vector<double>> gradient(const std::vector<double>& t, const std::vector<double>& inputs) {
+        need(t, inputs);
+        std::vector<double> g(nparam_);
+        double e = 0.0;
+        py::gil_scoped_release rel;
+        check(qfe_template_gradient(t_, t.data(), inputs.data(), g.data(), &e), "SectorTemplate.gradient");
+        return {e, g};
+    }
+    std::vector<double> pool_gradients(const std::vector<double>& t, const std::vector<QubitOperator>& ops, size_t batch,
+                                       const std::vector<double>& inputs) {
+        need(t, inputs);
+        std::vector<qfe_pauli> terms, one;
+        std::vector<uint64_t> off{0};
+        for (const auto& op : ops) {
+            if (!compile_pauli_sum(op, one)) throw std::invalid_argument("SectorTemplate.pool_gradients: operators must be Pauli sums");
+            terms.insert(terms.end(), one.begin(), one.end());
+            off.push_back(terms.size());
+        }
+        std::vector<double> out(ops.size());
+        py::gil_scoped_release rel;
+        check(qfe_template_pool_gradients(t_, t.data(), inputs.data(), terms.data(), off.data(), ops.size(), batch,
+                                          out.data()),
+              "SectorTemplate.pool_gradients");
+        return out;
+    }
     uint64_t dim() const {
         uint64_t d = 0;
         qfe_template_info(t_, &d, nullptr);
@@ -585,6 +610,9 @@ PYBIND11_MODULE(_core, m) {
              "nqubit"_a, "ref_index"_a, "generators"_a, "param_index"_a, "scale"_a, "nparam"_a, "hamiltonian_map"_a)
         .def("energy", &SectorTemplate::energy, "amplitudes"_a, "inputs"_a = std::vector<double>())
         .def("prepare", &SectorTemplate::prepare)
+        .def("gradient", &SectorTemplate::gradient, "amplitudes"_a, "inputs"_a = std::vector<double>())
+        .def("pool_gradients", &SectorTemplate::pool_gradients, "amplitudes"_a, "operators"_a, "batch"_a = 64,
+             "inputs"_a = std::vector<double>())
         .def_property_readonly("dim", &SectorTemplate::dim)
         .def_property_readonly("request_bytes", &SectorTemplate::request_bytes)
         .def_property_readonly("nqubit", &SectorTemplate::nqubit)

@@ -1,5 +1,6 @@
-// Determinant-sector engine: UCC energies on the (N_alpha, N_beta) sector of
-// the reference determinant, and the continuous-batching serving engine.
+// Determinant-sector engine: UCC energies, adjoint gradients and pool
+// gradients on the (N_alpha, N_beta) sector of the reference determinant, and
+// the continuous-batching serving engine built on them.
 //
 // A UCC generator K = sum_t kappa_t P_t built from a fermionic excitation
 // (JW of tau - tau^dag, ansatz.py:25-82 feeding trotterize) is a set of Pauli
@@ -20,8 +21,12 @@
 // pair (p, p ^ x).  The determinants of an entry are a product of an alpha
 // string list and a beta string list (with partner ranks precomputed).
 //
-// <H> = sum_groups sum_{j, j^x in sector} conj(psi_j) psi_{j^x}
-//       sum_t c_t (-1)^{(j^x).z_t}   (computer.cc:850-911 restated on the sector)
+// <H> = sum_groups sum_{j, j^x in sector} conj(psi_j) psi_{j^x} T_g(j^x),
+// T_g(j') = sum_{t in g} c_t (-1)^{j'.z_t}   (computer.cc:850-911 on the sector).
+// For x != 0 the pairs (j, j^x) and (j^x, j) contribute complex-conjugate
+// values (H_g is Hermitian), so only the half with the top bit of x clear in j
+// is visited, with weight 2.  The pair space of every group is cut into work
+// items of similar cost that run as independent workgroups over the chip.
 //
 // Layout: rank(j) = rank_alpha(alpha string) * D_beta + rank_beta(beta string),
 // strings ranked in increasing numeric order; alpha = even bits, beta = odd
@@ -42,12 +47,10 @@
 namespace qfe {
 namespace {
 
-constexpr int kSecThreads = 512;
-
 struct SecStep {  // one run of commuting exponentials sharing an X mask
     uint32_t e_first, e_count;
     int32_t param;
-    uint32_t pad;
+    uint32_t gen;
     double scale;
 };
 
@@ -61,9 +64,14 @@ struct SecEntry {  // one class of determinant pairs of a run
 
 struct SecGroup {  // Hamiltonian terms sharing an X mask, on sector pairs
     uint64_t x;
-    uint32_t la_off, la_len, lb_off, lb_len;
+    uint32_t la_off, la_len, lb_off, lb_len;  // half lists (x != 0) or full lists (x == 0)
     uint32_t t_first, t_count;
-    double inv_lb;
+    uint32_t xa, xb;                          // compressed flips
+    double inv_lb, weight;
+};
+
+struct SecItem {  // groups [g0, g1) whole, or pairs [p0, p1) of group g0 when g1 == g0 + 1
+    uint32_t g0, g1, p0, p1;
 };
 
 struct SecTables {  // device view of a template (lives in device memory)
@@ -71,30 +79,46 @@ struct SecTables {  // device view of a template (lives in device memory)
     uint32_t Da, Db;
     const uint64_t* afull;
     const uint64_t* bfull;
+    const uint32_t* astr;
+    const uint32_t* bstr;
+    const int32_t* arank;  // 2^norb entries, -1 outside the sector
+    const int32_t* brank;
     const int2* lists;
     const SecStep* steps;
     const SecEntry* entries;
     const SecGroup* groups;
+    const SecItem* items;
+    const uint32_t* gen_step;  // steps of generator k: [gen_step[k], gen_step[k+1])
     const uint64_t* tz;
     const double2* tph;    // phase of each term
-    const double2* tcoef;  // phase * coeff (fixed Hamiltonian) or null
+    const double2* tcoef;  // phase * coeff (fixed Hamiltonian)
     const uint64_t* map_off;
     const uint32_t* map_idx;
     const double2* map_val;
-    uint32_t nsteps, nentries, ngroups, nterms;
+    uint32_t nsteps, nentries, ngroups, nterms, nitems, ngen;
 };
 
 struct SlotDesc {
     const SecTables* t;
     double2* psi;
-    double2* coef;  // per-request coefficients (null: use t->tcoef)
+    double2* sig;   // H psi (gradients only)
+    double2* coef;  // per-request coefficients (null: t->tcoef)
     double2* cs;    // (cos, sin) per entry
+    double* part;   // per work item energy partials
+    double* gacc;   // per generator <sigma|K|psi> (gradients only)
     const double* params;
     const double* inputs;
     double* out;
-    uint32_t mode;  // 0: energy, 1: state only
-    uint32_t pad;
 };
+
+struct PoolOp {  // entries [e0, e1) of one pool operator
+    uint32_t e0, e1;
+};
+
+constexpr uint32_t kTableLds = 1024;  // alpha + beta string tables staged in LDS when they fit
+constexpr uint32_t kTermLds = 1024;   // Hamiltonian terms of one group staged in LDS when they fit
+constexpr int kUnroll = 4;            // independent pairs in flight per thread
+constexpr int kItemThreads = 256;
 
 __device__ __forceinline__ void divmod_u(uint32_t i, uint32_t d, double inv, uint32_t& q, uint32_t& r) {
     int64_t qq = (int64_t)((double)i * inv);
@@ -110,17 +134,25 @@ __device__ __forceinline__ void divmod_u(uint32_t i, uint32_t d, double inv, uin
     r = (uint32_t)rr;
 }
 
-// One workgroup per request: coefficients, angles, U|ref>, then <H>.
-__global__ __launch_bounds__(kSecThreads) void k_sector(const SlotDesc* __restrict__ slots) {
-    __shared__ double2 red[16];
+// Largest s with prefix[s] <= b (prefix[0] = 0, n + 1 entries).
+__device__ __forceinline__ uint32_t find_slot(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t b) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ---- 1. prologue: per-request Hamiltonian coefficients and rotation angles --------------
+__global__ __launch_bounds__(256) void k_sec_prologue(const SlotDesc* __restrict__ slots, int need_coef) {
     const SlotDesc sd = slots[blockIdx.x];
     const SecTables* __restrict__ T = sd.t;
-    const uint32_t tid = threadIdx.x, B = blockDim.x;
-    const uint64_t Db = T->Db;
-
-    const double2* coef = sd.coef ? sd.coef : T->tcoef;
-    if (sd.coef && sd.mode == 0) {
-        for (uint32_t k = tid; k < T->nterms; k += B) {
+    if (need_coef && sd.coef) {
+        for (uint32_t k = threadIdx.x; k < T->nterms; k += blockDim.x) {
             double cr = 0.0, ci = 0.0;
             const uint64_t e1 = T->map_off[k + 1];
             for (uint64_t e = T->map_off[k]; e < e1; ++e) {
@@ -132,75 +164,303 @@ __global__ __launch_bounds__(kSecThreads) void k_sector(const SlotDesc* __restri
             sd.coef[k] = cmul(T->tph[k], make_double2(cr, ci));
         }
     }
-    for (uint32_t e = tid; e < T->nentries; e += B) {
+    for (uint32_t e = threadIdx.x; e < T->nentries; e += blockDim.x) {
         const SecEntry& en = T->entries[e];
         const SecStep& st = T->steps[en.step];
-        const double th = st.scale * sd.params[st.param] * en.w;
         double s, c;
-        sincos(th, &s, &c);
+        sincos(st.scale * sd.params[st.param] * en.w, &s, &c);
         sd.cs[e] = make_double2(c, s);
     }
-    double2* __restrict__ psi = sd.psi;
-    for (uint64_t r = tid; r < T->D; r += B) psi[r] = make_double2(r == T->ref_rank ? 1.0 : 0.0, 0.0);
-    __syncthreads();
+}
 
-    for (uint32_t s = 0; s < T->nsteps; ++s) {
-        const uint32_t e0 = T->steps[s].e_first, e1 = e0 + T->steps[s].e_count;
-        for (uint32_t e = e0; e < e1; ++e) {
-            const SecEntry& en = T->entries[e];
-            const double2 cs = sd.cs[e];
-            const uint32_t lb = en.lb_len, total = en.la_len * lb;
-            const uint64_t x = en.x, z0 = en.z0;
-            const double2 ulo = en.ulo, uhi = en.uhi;
-            const double inv = en.inv_lb;
-            for (uint32_t i = tid; i < total; i += B) {
+// Rotate every pair of entry `en` by (c, sgn * s); pairs of one step are
+// disjoint, so kUnroll of them are in flight per thread (loads first).
+template <int THREADS>
+__device__ __forceinline__ void rotate_entry(const SecTables* __restrict__ T, const SecEntry& en, double c, double s,
+                                             double2* __restrict__ psi, const uint64_t* __restrict__ af,
+                                             const uint64_t* __restrict__ bf, uint32_t tid) {
+    const uint64_t Db = T->Db;
+    const uint32_t lb = en.lb_len, total = en.la_len * lb;
+    const uint64_t x = en.x, z0 = en.z0;
+    const double2 ulo = en.ulo, uhi = en.uhi;
+    const double inv = en.inv_lb;
+    for (uint32_t i0 = tid; i0 < total; i0 += kUnroll * THREADS) {
+        uint64_t r1[kUnroll], r2[kUnroll], j1[kUnroll];
+        double2 a[kUnroll], b[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            const uint32_t i = i0 + u * THREADS;
+            if (i < total) {
                 uint32_t ia, ib;
                 divmod_u(i, lb, inv, ia, ib);
                 const int2 pa = T->lists[en.la_off + ia];
                 const int2 pb = T->lists[en.lb_off + ib];
-                const uint64_t r1 = (uint64_t)pa.x * Db + (uint64_t)pb.x;
-                const uint64_t r2 = (uint64_t)pa.y * Db + (uint64_t)pb.y;
-                const uint64_t j1 = T->afull[pa.x] | T->bfull[pb.x];
-                const uint64_t j2 = j1 ^ x;
-                const double s1 = cs.y * parity_sign(j1 & z0), s2 = cs.y * parity_sign(j2 & z0);
-                const double2 a = psi[r1], b = psi[r2];
-                // psi_1' = c a + s s(j2) uhi b ; psi_2' = c b + s s(j1) ulo a
-                const double2 hb = cmul(uhi, b), la = cmul(ulo, a);
-                psi[r1] = make_double2(cs.x * a.x + s2 * hb.x, cs.x * a.y + s2 * hb.y);
-                psi[r2] = make_double2(cs.x * b.x + s1 * la.x, cs.x * b.y + s1 * la.y);
+                r1[u] = (uint64_t)pa.x * Db + (uint64_t)pb.x;
+                r2[u] = (uint64_t)pa.y * Db + (uint64_t)pb.y;
+                j1[u] = af[pa.x] | bf[pb.x];
             }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (i0 + u * THREADS < total) {
+                a[u] = psi[r1[u]];
+                b[u] = psi[r2[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+            if (i0 + u * THREADS < total) {
+                const double s1 = s * parity_sign(j1[u] & z0), s2 = s * parity_sign((j1[u] ^ x) & z0);
+                // psi_1' = c a + s s(j2) uhi b ; psi_2' = c b + s s(j1) ulo a
+                const double2 hb = cmul(uhi, b[u]), la = cmul(ulo, a[u]);
+                psi[r1[u]] = make_double2(c * a[u].x + s2 * hb.x, c * a[u].y + s2 * hb.y);
+                psi[r2[u]] = make_double2(c * b[u].x + s1 * la.x, c * b[u].y + s1 * la.y);
+            }
+        }
+    }
+}
+
+// Stage the string tables into LDS when they fit; returns the table pointers.
+__device__ __forceinline__ void stage_tables(const SecTables* __restrict__ T, uint64_t* tab, const uint64_t*& af,
+                                             const uint64_t*& bf, uint32_t tid, uint32_t nthreads) {
+    if (T->Da + T->Db <= kTableLds) {
+        for (uint32_t k = tid; k < T->Da; k += nthreads) tab[k] = T->afull[k];
+        for (uint32_t k = tid; k < T->Db; k += nthreads) tab[T->Da + k] = T->bfull[k];
+        af = tab;
+        bf = tab + T->Da;
+    } else {
+        af = T->afull;
+        bf = T->bfull;
+    }
+}
+
+// ---- 2. ansatz: U|ref> by one workgroup per request ------------------------------------
+// With PSI_LDS the state lives in LDS (dynamic LDS = D * 16 bytes + string
+// tables) and is written to the request's buffer at the end.
+template <bool PSI_LDS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sec_ansatz(const SlotDesc* __restrict__ slots) {
+    extern __shared__ double2 sh[];
+    const SlotDesc sd = slots[blockIdx.x];
+    const SecTables* __restrict__ T = sd.t;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t D = T->D;
+    double2* __restrict__ psi = PSI_LDS ? sh : sd.psi;
+    const uint64_t *af, *bf;
+    stage_tables(T, (uint64_t*)(sh + (PSI_LDS ? D : 0)), af, bf, tid, THREADS);
+    for (uint64_t r = tid; r < D; r += THREADS) psi[r] = make_double2(r == T->ref_rank ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+    for (uint32_t s = 0; s < T->nsteps; ++s) {
+        const uint32_t e0 = T->steps[s].e_first, e1 = e0 + T->steps[s].e_count;
+        for (uint32_t e = e0; e < e1; ++e) {
+            const double2 cs = sd.cs[e];
+            rotate_entry<THREADS>(T, T->entries[e], cs.x, cs.y, psi, af, bf, tid);
         }
         __syncthreads();
     }
-    if (sd.mode != 0) return;
+    if (PSI_LDS)
+        for (uint64_t r = tid; r < D; r += THREADS) sd.psi[r] = psi[r];
+}
 
+// ---- 3. energy work items: one workgroup per (request, item) ---------------------------
+__global__ __launch_bounds__(kItemThreads) void k_sec_energy(const SlotDesc* __restrict__ slots,
+                                                             const uint32_t* __restrict__ prefix, uint32_t nslots) {
+    __shared__ uint64_t tz_s[kTermLds];
+    __shared__ double2 c_s[kTermLds];
+    __shared__ double2 red[16];
+    const uint32_t slot = find_slot(prefix, nslots, blockIdx.x);
+    const SlotDesc sd = slots[slot];
+    const SecTables* __restrict__ T = sd.t;
+    const SecItem it = T->items[blockIdx.x - prefix[slot]];
+    const double2* __restrict__ coef = sd.coef ? sd.coef : T->tcoef;
+    const double2* __restrict__ psi = sd.psi;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t Db = T->Db;
+    const bool whole = it.g1 != it.g0 + 1;
     double acc = 0.0;
+    for (uint32_t g = it.g0; g < it.g1; ++g) {
+        const SecGroup& gr = T->groups[g];
+        const uint32_t t0 = gr.t_first, nt = gr.t_count;
+        const bool lds = nt <= kTermLds;
+        if (lds) {
+            __syncthreads();
+            for (uint32_t k = tid; k < nt; k += kItemThreads) {
+                tz_s[k] = T->tz[t0 + k];
+                c_s[k] = coef[t0 + k];
+            }
+            __syncthreads();
+        }
+        const uint32_t lb = gr.lb_len;
+        const uint32_t p0 = whole ? 0u : it.p0, p1 = whole ? gr.la_len * lb : it.p1;
+        const uint64_t x = gr.x;
+        double gacc = 0.0;
+        for (uint32_t i0 = p0 + tid; i0 < p1; i0 += kUnroll * kItemThreads) {
+            uint64_t j2[kUnroll];
+            double2 v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const uint32_t i = i0 + u * kItemThreads;
+                j2[u] = 0;
+                v[u] = make_double2(0.0, 0.0);
+                if (i < p1) {
+                    uint32_t ia, ib;
+                    divmod_u(i, lb, gr.inv_lb, ia, ib);
+                    const int2 pa = T->lists[gr.la_off + ia];
+                    const int2 pb = T->lists[gr.lb_off + ib];
+                    j2[u] = (T->afull[pa.x] | T->bfull[pb.x]) ^ x;
+                    v[u] = cconjmul(psi[(uint64_t)pa.x * Db + (uint64_t)pb.x], psi[(uint64_t)pa.y * Db + (uint64_t)pb.y]);
+                }
+            }
+            double tr[kUnroll], ti[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) tr[u] = ti[u] = 0.0;
+            for (uint32_t t = 0; t < nt; ++t) {
+                const uint64_t z = lds ? tz_s[t] : T->tz[t0 + t];
+                const double2 c = lds ? c_s[t] : coef[t0 + t];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const double sg = parity_sign(j2[u] & z);
+                    tr[u] += sg * c.x;
+                    ti[u] += sg * c.y;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) gacc += v[u].x * tr[u] - v[u].y * ti[u];
+        }
+        acc += gr.weight * gacc;
+    }
+    const double2 tot = block_sum(make_double2(acc, 0.0), red);
+    if (tid == 0) sd.part[blockIdx.x - prefix[slot]] = tot.x;
+}
+
+// ---- 4. per-request sum of the item partials in a fixed order ---------------------------
+__global__ __launch_bounds__(256) void k_sec_sum(const SlotDesc* __restrict__ slots) {
+    __shared__ double2 red[16];
+    const SlotDesc sd = slots[blockIdx.x];
+    const uint32_t n = sd.t->nitems;
+    double acc = 0.0;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) acc += sd.part[k];
+    const double2 tot = block_sum(make_double2(acc, 0.0), red);
+    if (threadIdx.x == 0) *sd.out = tot.x;
+}
+
+// ---- gradients ---------------------------------------------------------------------------
+// sigma = H psi, output-centric: each determinant gathers from every group
+// whose flip keeps it in the sector.
+__global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ slots) {
+    const SlotDesc sd = slots[blockIdx.y];
+    const SecTables* __restrict__ T = sd.t;
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= T->D) return;
+    const uint32_t ra = (uint32_t)(r / T->Db), rb = (uint32_t)(r % T->Db);
+    const uint32_t sa = T->astr[ra], sb = T->bstr[rb];
+    const uint64_t j = T->afull[ra] | T->bfull[rb];
+    const double2* __restrict__ coef = sd.coef ? sd.coef : T->tcoef;
+    double sr = 0.0, si = 0.0;
     for (uint32_t g = 0; g < T->ngroups; ++g) {
         const SecGroup& gr = T->groups[g];
-        const uint32_t lb = gr.lb_len, total = gr.la_len * lb;
-        const uint64_t x = gr.x;
-        const uint32_t t0 = gr.t_first, t1 = gr.t_first + gr.t_count;
-        const double inv = gr.inv_lb;
-        for (uint32_t i = tid; i < total; i += B) {
-            uint32_t ia, ib;
-            divmod_u(i, lb, inv, ia, ib);
-            const int2 pa = T->lists[gr.la_off + ia];
-            const int2 pb = T->lists[gr.lb_off + ib];
-            const uint64_t r1 = (uint64_t)pa.x * Db + (uint64_t)pb.x;
-            const uint64_t r2 = (uint64_t)pa.y * Db + (uint64_t)pb.y;
-            const uint64_t j2 = (T->afull[pa.x] | T->bfull[pb.x]) ^ x;
-            double tr = 0.0, ti = 0.0;
-            for (uint32_t t = t0; t < t1; ++t) {
-                const double sg = parity_sign(j2 & T->tz[t]);
-                tr += sg * coef[t].x;
-                ti += sg * coef[t].y;
+        if (2 * __popc(sa & gr.xa) != __popc(gr.xa) || 2 * __popc(sb & gr.xb) != __popc(gr.xb)) continue;
+        const uint64_t r2 = (uint64_t)T->arank[sa ^ gr.xa] * T->Db + (uint64_t)T->brank[sb ^ gr.xb];
+        const uint64_t j2 = j ^ gr.x;
+        double tr = 0.0, ti = 0.0;
+        for (uint32_t t = gr.t_first; t < gr.t_first + gr.t_count; ++t) {
+            const double sg = parity_sign(j2 & T->tz[t]);
+            tr += sg * coef[t].x;
+            ti += sg * coef[t].y;
+        }
+        const double2 p = sd.psi[r2];
+        sr += tr * p.x - ti * p.y;
+        si += tr * p.y + ti * p.x;
+    }
+    sd.sig[r] = make_double2(sr, si);
+}
+
+// Sum over the pairs of entry en of Re(conj(sig) K psi).
+template <int THREADS>
+__device__ __forceinline__ double entry_overlap(const SecTables* __restrict__ T, const SecEntry& en,
+                                                const double2* __restrict__ sig, const double2* __restrict__ psi,
+                                                uint32_t tid) {
+    const uint64_t Db = T->Db;
+    const uint32_t lb = en.lb_len, total = en.la_len * lb;
+    double acc = 0.0;
+    for (uint32_t i = tid; i < total; i += THREADS) {
+        uint32_t ia, ib;
+        divmod_u(i, lb, en.inv_lb, ia, ib);
+        const int2 pa = T->lists[en.la_off + ia];
+        const int2 pb = T->lists[en.lb_off + ib];
+        const uint64_t r1 = (uint64_t)pa.x * Db + (uint64_t)pb.x, r2 = (uint64_t)pa.y * Db + (uint64_t)pb.y;
+        const uint64_t j1 = T->afull[pa.x] | T->bfull[pb.x];
+        const double s1 = en.w * parity_sign(j1 & en.z0), s2 = en.w * parity_sign((j1 ^ en.x) & en.z0);
+        // (K psi)_1 = s(j2) w uhi psi_2, (K psi)_2 = s(j1) w ulo psi_1
+        const double2 k1 = cmul(en.uhi, psi[r2]), k2 = cmul(en.ulo, psi[r1]);
+        const double2 a = sig[r1], b = sig[r2];
+        acc += s2 * (a.x * k1.x + a.y * k1.y) + s1 * (b.x * k2.x + b.y * k2.y);
+    }
+    return acc;
+}
+
+// Reverse sweep (one workgroup per request): E = Re<psi|sigma>, then for
+// k = M-1 .. 0: <sigma_k|K_k|psi_k>, and un-apply generator k on psi and sigma.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sec_reverse(const SlotDesc* __restrict__ slots) {
+    __shared__ double2 red[16];
+    const SlotDesc sd = slots[blockIdx.x];
+    const SecTables* __restrict__ T = sd.t;
+    const uint32_t tid = threadIdx.x;
+    double2* __restrict__ psi = sd.psi;
+    double2* __restrict__ sig = sd.sig;
+    double e = 0.0;
+    for (uint64_t r = tid; r < T->D; r += THREADS) e += psi[r].x * sig[r].x + psi[r].y * sig[r].y;
+    const double2 et = block_sum(make_double2(e, 0.0), red);
+    if (tid == 0) *sd.out = et.x;
+    for (uint32_t k = T->ngen; k-- > 0;) {
+        const uint32_t s0 = T->gen_step[k], s1 = T->gen_step[k + 1];
+        double acc = 0.0;
+        for (uint32_t s = s0; s < s1; ++s)
+            for (uint32_t en = T->steps[s].e_first; en < T->steps[s].e_first + T->steps[s].e_count; ++en)
+                acc += entry_overlap<THREADS>(T, T->entries[en], sig, psi, tid);
+        const double2 tot = block_sum(make_double2(acc, 0.0), red);  // its barriers end every read above
+        if (tid == 0) sd.gacc[k] = tot.x;
+        if (k == 0) break;
+        for (uint32_t s = s1; s-- > s0;) {
+            const uint32_t e0 = T->steps[s].e_first, e1 = e0 + T->steps[s].e_count;
+            for (uint32_t en = e0; en < e1; ++en) {
+                const double2 cs = sd.cs[en];
+                rotate_entry<THREADS>(T, T->entries[en], cs.x, -cs.y, psi, T->afull, T->bfull, tid);
+                rotate_entry<THREADS>(T, T->entries[en], cs.x, -cs.y, sig, T->afull, T->bfull, tid);
             }
-            const double2 v = cconjmul(psi[r1], psi[r2]);
-            acc += v.x * tr - v.y * ti;
+            __syncthreads();
+        }
+    }
+}
+
+// Pool gradients: 2 Re<H psi|A_k psi> for the operators of one batch, one
+// workgroup per operator (entries and lists of the pool's own tables).
+__global__ __launch_bounds__(256) void k_sec_pool(const SecTables* __restrict__ T, const SecEntry* __restrict__ pe,
+                                                  const int2* __restrict__ plists, const PoolOp* __restrict__ ops,
+                                                  const double2* __restrict__ sig, const double2* __restrict__ psi,
+                                                  double* __restrict__ out) {
+    __shared__ double2 red[16];
+    const PoolOp op = ops[blockIdx.x];
+    const uint64_t Db = T->Db;
+    double acc = 0.0;
+    for (uint32_t e = op.e0; e < op.e1; ++e) {
+        const SecEntry en = pe[e];
+        const uint32_t lb = en.lb_len, total = en.la_len * lb;
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            uint32_t ia, ib;
+            divmod_u(i, lb, en.inv_lb, ia, ib);
+            const int2 pa = plists[en.la_off + ia];
+            const int2 pb = plists[en.lb_off + ib];
+            const uint64_t r1 = (uint64_t)pa.x * Db + (uint64_t)pb.x, r2 = (uint64_t)pa.y * Db + (uint64_t)pb.y;
+            const uint64_t j1 = T->afull[pa.x] | T->bfull[pb.x];
+            const double s1 = en.w * parity_sign(j1 & en.z0), s2 = en.w * parity_sign((j1 ^ en.x) & en.z0);
+            const double2 k1 = cmul(en.uhi, psi[r2]), k2 = cmul(en.ulo, psi[r1]);
+            const double2 a = sig[r1], b = sig[r2];
+            acc += s2 * (a.x * k1.x + a.y * k1.y) + s1 * (b.x * k2.x + b.y * k2.y);
         }
     }
     const double2 tot = block_sum(make_double2(acc, 0.0), red);
-    if (tid == 0) *sd.out = tot.x;
+    if (threadIdx.x == 0) out[blockIdx.x] = 2.0 * tot.x;
 }
 
 // Scatter a sector state into a zeroed full state.
@@ -211,7 +471,22 @@ __global__ void k_sector_scatter(const double2* __restrict__ psi, uint64_t D, ui
     out[afull[r / Db] | bfull[r % Db]] = psi[r];
 }
 
-// ---- host-side construction -------------------------------------------------------
+// Ansatz launch shapes by size tier: 0 = state in LDS, 256 threads; 1 = state
+// in LDS, 512 threads; 2 = state in device memory, 1024 threads.
+constexpr size_t kTier0Lds = 40 << 10;
+
+int launch_ansatz(int tier, unsigned nblocks, size_t lds, hipStream_t st, const SlotDesc* d) {
+    if (tier == 0)
+        hipLaunchKernelGGL((k_sec_ansatz<true, 256>), dim3(nblocks), dim3(256), lds, st, d);
+    else if (tier == 1)
+        hipLaunchKernelGGL((k_sec_ansatz<true, 512>), dim3(nblocks), dim3(512), lds, st, d);
+    else
+        hipLaunchKernelGGL((k_sec_ansatz<false, 1024>), dim3(nblocks), dim3(1024), lds, st, d);
+    QFE_HIP(hipGetLastError());
+    return QFE_OK;
+}
+
+// ---- host-side construction -------------------------------------------------------------
 
 inline uint32_t compress(uint64_t v, int parity, int norb) {
     uint32_t out = 0;
@@ -225,6 +500,9 @@ inline uint64_t spread(uint32_t v, int parity, int norb) {
 }
 inline int popc32(uint32_t v) { return __builtin_popcount(v); }
 inline int popc64(uint64_t v) { return __builtin_popcountll(v); }
+inline double2 cplx_mul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+inline double cplx_abs(double2 a) { return std::hypot(a.x, a.y); }
+inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct Space {
     int norb = 0;
@@ -245,22 +523,27 @@ struct Space {
     }
 };
 
+// Lists of (rank, partner rank) string pairs.  kind 0: (a & mask) == pattern;
+// kind 1: popcount(a & mask) == popcount(mask) / 2; kind 2: kind 1 and
+// (a & clear) == 0.  The partner is a ^ flip.
 struct ListPool {
     std::vector<int2> pool;
-    std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> index;
-    // strings a of side s with (a & mask) == pattern (pattern == ~0u: popcount(a & mask) == popcount(mask) / 2),
-    // each paired with the rank of a ^ flip.
-    std::pair<uint32_t, uint32_t> get(const Space& sp, int s, uint32_t mask, uint32_t pattern, uint32_t flip) {
-        auto key = std::make_tuple(s, mask, pattern, flip);
+    std::map<std::tuple<int, int, uint32_t, uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> index;
+    std::pair<uint32_t, uint32_t> get(const Space& sp, int s, int kind, uint32_t mask, uint32_t pattern, uint32_t flip) {
+        auto key = std::make_tuple(s, kind, mask, pattern, flip);
         auto it = index.find(key);
         if (it != index.end()) return it->second;
         const uint32_t off = (uint32_t)pool.size();
         for (size_t r = 0; r < sp.str[s].size(); ++r) {
             const uint32_t a = sp.str[s][r];
-            const bool ok = pattern == ~0u ? (2 * popc32(a & mask) == popc32(mask)) : ((a & mask) == pattern);
+            bool ok;
+            if (kind == 0)
+                ok = (a & mask) == pattern;
+            else
+                ok = 2 * popc32(a & mask) == popc32(mask) && (kind == 1 || (a & pattern) == 0);
             if (!ok) continue;
             const int32_t partner = sp.rank[s][a ^ flip];
-            if (partner < 0) continue;  // cannot happen for conserving entries; keep lists exact
+            if (partner < 0) continue;
             pool.push_back(make_int2((int)r, partner));
         }
         auto v = std::make_pair(off, (uint32_t)pool.size() - off);
@@ -269,8 +552,107 @@ struct ListPool {
     }
 };
 
-inline double2 cplx_mul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-inline double cplx_abs(double2 a) { return std::hypot(a.x, a.y); }
+// Split generator k (terms [b, e)) into runs and entries; appends to steps
+// and entries.  Used for ansatz generators and for pool operators.
+int build_runs(const Space& sp, ListPool& lp, int norb, uint64_t lim, const qfe_pauli* terms, uint64_t b, uint64_t e,
+               int32_t param, double scale, uint32_t gen, std::vector<SecStep>& steps, std::vector<SecEntry>& entries,
+               const char* who) {
+    while (b < e) {
+        const uint64_t x = terms[b].x, z0 = terms[b].z;
+        uint64_t r = b;
+        while (r < e && terms[r].x == x && (popc64(x & (terms[r].z ^ z0)) & 1) == 0) ++r;
+        uint64_t m = x;
+        for (uint64_t k = b; k < r; ++k) {
+            if ((terms[k].x | terms[k].z) & ~lim) return fail(QFE_ERR_ARG, std::string(who) + ": term acts on a qubit out of range");
+            m |= terms[k].z ^ z0;
+        }
+        if (popc64(m) > 20) return fail(QFE_ERR_ARG, std::string(who) + ": a run of commuting terms spans more than 20 qubits");
+        double scale_abs = 0.0;
+        std::vector<double2> kap(r - b);
+        for (uint64_t k = b; k < r; ++k) {
+            // exp(theta * coeff * P) with coeff purely imaginary, as the full-space program applies it
+            kap[k - b] = cplx_mul(make_double2(0.0, terms[k].coeff.im), make_double2(terms[k].phase.re, terms[k].phase.im));
+            scale_abs += cplx_abs(kap[k - b]);
+        }
+        const double tol = 1e-13 * (scale_abs > 0 ? scale_abs : 1.0);
+        std::map<uint64_t, double2> A;  // A(p) over the submasks of m
+        for (uint64_t p = m;; p = (p - 1) & m) {
+            double2 s = make_double2(0.0, 0.0);
+            for (uint64_t k = b; k < r; ++k) {
+                const double sg = (popc64(p & (terms[k].z ^ z0)) & 1) ? -1.0 : 1.0;
+                s.x += sg * kap[k - b].x;
+                s.y += sg * kap[k - b].y;
+            }
+            if (cplx_abs(s) > tol) A.emplace(p, s);
+            if (p == 0) break;
+        }
+        if (x == 0 && !A.empty()) return fail(QFE_ERR_ARG, std::string(who) + ": diagonal runs are not supported on the sector");
+        SecStep st{};
+        st.e_first = (uint32_t)entries.size();
+        st.param = param;
+        st.scale = scale;
+        st.gen = gen;
+        const double sx = (popc64(x & z0) & 1) ? -1.0 : 1.0;
+        for (const auto& kv : A) {
+            const uint64_t plo = kv.first, phi = plo ^ x;
+            auto it = A.find(phi);
+            if (it == A.end()) return fail(QFE_ERR_ARG, std::string(who) + ": a run is not anti-Hermitian");
+            if (plo > phi) continue;
+            const double2 alo = kv.second, ahi = it->second;
+            const double w = cplx_abs(alo);
+            const double2 prod = cplx_mul(alo, ahi);
+            if (std::fabs(sx * prod.x + w * w) > 1e-12 * w * w || std::fabs(prod.y) > 1e-12 * w * w)
+                return fail(QFE_ERR_ARG, std::string(who) + ": a run is not anti-Hermitian");
+            const uint32_t pa = compress(plo, 0, norb), pb = compress(plo, 1, norb);
+            const uint32_t qa = compress(phi, 0, norb), qb = compress(phi, 1, norb);
+            if (popc32(pa) != popc32(qa) || popc32(pb) != popc32(qb))
+                return fail(QFE_ERR_ARG, std::string(who) + ": a run does not conserve the alpha and beta particle numbers");
+            const uint32_t ma = compress(m, 0, norb), mb = compress(m, 1, norb);
+            const uint32_t xa = compress(x, 0, norb), xb = compress(x, 1, norb);
+            auto la = lp.get(sp, 0, 0, ma, pa, xa);
+            auto lb = lp.get(sp, 1, 0, mb, pb, xb);
+            if (!la.second || !lb.second) continue;
+            SecEntry en{};
+            en.x = x;
+            en.z0 = z0;
+            en.la_off = la.first;
+            en.la_len = la.second;
+            en.lb_off = lb.first;
+            en.lb_len = lb.second;
+            en.w = w;
+            en.inv_lb = 1.0 / (double)lb.second;
+            en.ulo = make_double2(alo.x / w, alo.y / w);
+            en.uhi = make_double2(ahi.x / w, ahi.y / w);
+            en.step = (uint32_t)steps.size();
+            entries.push_back(en);
+        }
+        st.e_count = (uint32_t)entries.size() - st.e_first;
+        if (st.e_count) steps.push_back(st);
+        b = r;
+    }
+    return QFE_OK;
+}
+
+// Packs host arrays into one device allocation at 256-byte offsets.
+struct Packer {
+    struct Piece {
+        const void* src;
+        size_t bytes, off;
+    };
+    std::vector<Piece> pieces;
+    size_t total;
+    explicit Packer(size_t head) : total(align256(head)) {}
+    size_t add(const void* src, size_t bytes) {
+        pieces.push_back({src, bytes, total});
+        total += align256(bytes ? bytes : 1);
+        return pieces.size() - 1;
+    }
+    void fill(std::vector<char>& img) const {
+        img.assign(total, 0);
+        for (const auto& p : pieces)
+            if (p.bytes) std::memcpy(img.data() + p.off, p.src, p.bytes);
+    }
+};
 
 }  // namespace
 }  // namespace qfe
@@ -281,24 +663,27 @@ struct qfe_template_s {
     int n = 0, norb = 0;
     uint64_t D = 0;
     uint32_t Da = 0, Db = 0;
-    uint64_t nparam = 0, ninputs = 0;
-    uint32_t nterms = 0, nentries = 0;
+    uint64_t nparam = 0, ninputs = 0, ngen = 0;
+    std::vector<int32_t> gparam;
+    std::vector<double> gscale;
+    uint32_t nterms = 0, nentries = 0, nitems = 0;
     bool has_map = false;
-    uint64_t cost = 0;           // work estimate for ordering slots in a launch
-    void* dev = nullptr;         // all tables
-    SecTables* dt = nullptr;     // device struct inside dev
+    uint64_t cost = 0;     // work estimate for ordering slots in a launch
+    int tier = 2;          // ansatz launch shape
+    size_t lds_bytes = 0;  // dynamic LDS of the ansatz kernel
+    Space sp;              // host copy of the string space (pool operators)
+    void* dev = nullptr;   // all tables
+    SecTables* dt = nullptr;
     const uint64_t* afull = nullptr;
     const uint64_t* bfull = nullptr;
-    // per-request device bytes: work (psi, coef, cs) + io (params, inputs)
-    size_t off_psi = 0, off_coef = 0, off_cs = 0, work_bytes = 0, io_count = 0, request_bytes = 0;
-    // single-request arena
-    void* arena = nullptr;
+    // per-request device bytes: psi, coef, cs, part (energy); + sig, gacc (gradient)
+    size_t off_psi = 0, off_coef = 0, off_cs = 0, off_part = 0, off_sig = 0, off_gacc = 0;
+    size_t work_bytes = 0, grad_bytes = 0, io_count = 0, request_bytes = 0;
+    void* arena = nullptr;  // single-request arena: [desc][out][io][prefix][work incl. gradient parts]
 };
 
 namespace qfe {
 namespace {
-
-inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_terms, const uint64_t* gen_off, uint64_t ngen,
                    const int32_t* gen_param, const double* gen_scale, uint64_t nparam, const qfe_pauli* ham, uint64_t nham,
@@ -308,7 +693,7 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     const uint64_t lim = (1ull << n) - 1ull;
     if (ref & ~lim) return fail(QFE_ERR_ARG, "qfe_template_create: reference index out of range");
     const uint32_t refa = compress(ref, 0, norb), refb = compress(ref, 1, norb);
-    Space sp;
+    Space& sp = t->sp;
     sp.build(norb, popc32(refa), popc32(refb));
     ListPool lp;
     t->n = n;
@@ -318,96 +703,22 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     t->D = (uint64_t)t->Da * t->Db;
     t->nparam = nparam;
     t->ninputs = ninputs;
+    t->ngen = ngen;
+    t->gparam.assign(gen_param, gen_param + ngen);
+    t->gscale.assign(gen_scale, gen_scale + ngen);
     const uint64_t ref_rank = (uint64_t)sp.rank[0][refa] * t->Db + (uint64_t)sp.rank[1][refb];
 
-    // generators -> runs -> entries
     std::vector<SecStep> steps;
     std::vector<SecEntry> entries;
+    std::vector<uint32_t> gen_step{0};
     for (uint64_t g = 0; g < ngen; ++g) {
         if (gen_off[g + 1] < gen_off[g]) return fail(QFE_ERR_ARG, "qfe_template_create: generator offsets must not decrease");
         if (gen_param[g] < 0 || (uint64_t)gen_param[g] >= nparam)
             return fail(QFE_ERR_ARG, "qfe_template_create: generator parameter index out of range");
-        uint64_t b = gen_off[g];
-        const uint64_t e = gen_off[g + 1];
-        while (b < e) {
-            const uint64_t x = gen_terms[b].x, z0 = gen_terms[b].z;
-            uint64_t r = b;
-            while (r < e && gen_terms[r].x == x && (popc64(x & (gen_terms[r].z ^ z0)) & 1) == 0) ++r;
-            // run [b, r)
-            uint64_t m = x;
-            for (uint64_t k = b; k < r; ++k) {
-                if ((gen_terms[k].x | gen_terms[k].z) & ~lim)
-                    return fail(QFE_ERR_ARG, "qfe_template_create: generator term acts on a qubit out of range");
-                m |= gen_terms[k].z ^ z0;
-            }
-            if (popc64(m) > 20) return fail(QFE_ERR_ARG, "qfe_template_create: generator run spans more than 20 qubits");
-            double scale_abs = 0.0;
-            std::vector<double2> kap(r - b);
-            for (uint64_t k = b; k < r; ++k) {
-                // exp(theta * coeff * P) with coeff purely imaginary, as the full-space program applies it
-                kap[k - b] = cplx_mul(make_double2(0.0, gen_terms[k].coeff.im),
-                                      make_double2(gen_terms[k].phase.re, gen_terms[k].phase.im));
-                scale_abs += cplx_abs(kap[k - b]);
-            }
-            const double tol = 1e-13 * (scale_abs > 0 ? scale_abs : 1.0);
-            // A(p) over the submasks of m
-            std::map<uint64_t, double2> A;
-            for (uint64_t p = m;; p = (p - 1) & m) {
-                double2 s = make_double2(0.0, 0.0);
-                for (uint64_t k = b; k < r; ++k) {
-                    const double sg = (popc64(p & (gen_terms[k].z ^ z0)) & 1) ? -1.0 : 1.0;
-                    s.x += sg * kap[k - b].x;
-                    s.y += sg * kap[k - b].y;
-                }
-                if (cplx_abs(s) > tol) A.emplace(p, s);
-                if (p == 0) break;
-            }
-            if (x == 0 && !A.empty())
-                return fail(QFE_ERR_ARG, "qfe_template_create: diagonal generator runs are not supported on the sector");
-            SecStep st{};
-            st.e_first = (uint32_t)entries.size();
-            st.param = gen_param[g];
-            st.scale = gen_scale[g];
-            const double sx = (popc64(x & z0) & 1) ? -1.0 : 1.0;
-            for (const auto& kv : A) {
-                const uint64_t plo = kv.first, phi = plo ^ x;
-                auto it = A.find(phi);
-                if (it == A.end())
-                    return fail(QFE_ERR_ARG, "qfe_template_create: a generator run is not anti-Hermitian");
-                if (plo > phi) continue;
-                const double2 alo = kv.second, ahi = it->second;
-                const double w = cplx_abs(alo);
-                const double2 prod = cplx_mul(alo, ahi);
-                if (std::fabs(sx * prod.x + w * w) > 1e-12 * w * w || std::fabs(prod.y) > 1e-12 * w * w)
-                    return fail(QFE_ERR_ARG, "qfe_template_create: a generator run is not anti-Hermitian");
-                const uint32_t pa = compress(plo, 0, norb), pb = compress(plo, 1, norb);
-                const uint32_t qa = compress(phi, 0, norb), qb = compress(phi, 1, norb);
-                if (popc32(pa) != popc32(qa) || popc32(pb) != popc32(qb))
-                    return fail(QFE_ERR_ARG,
-                                "qfe_template_create: a generator run does not conserve the alpha and beta particle numbers");
-                const uint32_t ma = compress(m, 0, norb), mb = compress(m, 1, norb);
-                const uint32_t xa = compress(x, 0, norb), xb = compress(x, 1, norb);
-                auto la = lp.get(sp, 0, ma, pa, xa);
-                auto lb = lp.get(sp, 1, mb, pb, xb);
-                if (!la.second || !lb.second) continue;
-                SecEntry en{};
-                en.x = x;
-                en.z0 = z0;
-                en.la_off = la.first;
-                en.la_len = la.second;
-                en.lb_off = lb.first;
-                en.lb_len = lb.second;
-                en.w = w;
-                en.inv_lb = 1.0 / (double)lb.second;
-                en.ulo = make_double2(alo.x / w, alo.y / w);
-                en.uhi = make_double2(ahi.x / w, ahi.y / w);
-                en.step = (uint32_t)steps.size();
-                entries.push_back(en);
-            }
-            st.e_count = (uint32_t)entries.size() - st.e_first;
-            if (st.e_count) steps.push_back(st);
-            b = r;
-        }
+        int rc = build_runs(sp, lp, norb, lim, gen_terms, gen_off[g], gen_off[g + 1], gen_param[g], gen_scale[g],
+                            (uint32_t)g, steps, entries, "qfe_template_create");
+        if (rc) return rc;
+        gen_step.push_back((uint32_t)steps.size());
     }
 
     // Hamiltonian: groups by X mask (first appearance), sector pairs only
@@ -430,11 +741,29 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
         for (uint64_t x : order) {
             const uint32_t xa = compress(x, 0, norb), xb = compress(x, 1, norb);
             if ((popc32(xa) & 1) || (popc32(xb) & 1)) continue;  // never connects two sector determinants
-            auto la = lp.get(sp, 0, xa, ~0u, xa);
-            auto lb = lp.get(sp, 1, xb, ~0u, xb);
-            if (!la.second || !lb.second) continue;
             SecGroup gr{};
             gr.x = x;
+            gr.xa = xa;
+            gr.xb = xb;
+            std::pair<uint32_t, uint32_t> la, lb;
+            if (x == 0) {
+                la = lp.get(sp, 0, 1, 0, 0, 0);
+                lb = lp.get(sp, 1, 1, 0, 0, 0);
+                gr.weight = 1.0;
+            } else {
+                // visit the half of the pairs with the top bit of x clear, weight 2
+                const int hb = 63 - __builtin_clzll(x);
+                const uint32_t clear = 1u << (hb / 2);
+                if ((hb & 1) == 0) {
+                    la = lp.get(sp, 0, 2, xa, clear, xa);
+                    lb = lp.get(sp, 1, 1, xb, 0, xb);
+                } else {
+                    la = lp.get(sp, 0, 1, xa, 0, xa);
+                    lb = lp.get(sp, 1, 2, xb, clear, xb);
+                }
+                gr.weight = 2.0;
+            }
+            if (!la.second || !lb.second) continue;
             gr.la_off = la.first;
             gr.la_len = la.second;
             gr.lb_off = lb.first;
@@ -460,76 +789,127 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
             groups.push_back(gr);
         }
     }
+    // work items of similar cost (pairs x (terms + 4)): small groups merge,
+    // large groups split into pair ranges
+    std::vector<SecItem> items;
+    {
+        const uint64_t target = 1ull << 18;
+        uint32_t g = 0;
+        while (g < groups.size()) {
+            const uint64_t pairs = (uint64_t)groups[g].la_len * groups[g].lb_len;
+            const uint64_t per = groups[g].t_count + 4;
+            if (pairs * per >= target) {
+                uint64_t chunk = std::max<uint64_t>(2048, target / per);
+                chunk = (chunk + 255) & ~255ull;
+                for (uint64_t p = 0; p < pairs; p += chunk)
+                    items.push_back(SecItem{g, g + 1, (uint32_t)p, (uint32_t)std::min(pairs, p + chunk)});
+                ++g;
+                continue;
+            }
+            uint32_t g1 = g;
+            uint64_t cost = 0;
+            while (g1 < groups.size()) {
+                const uint64_t c = (uint64_t)groups[g1].la_len * groups[g1].lb_len * (groups[g1].t_count + 4);
+                if (g1 > g && (cost + c > target || c >= target)) break;
+                cost += c;
+                ++g1;
+            }
+            // a single small group is "whole" only when g1 != g + 1; give it its full pair range instead
+            if (g1 == g + 1)
+                items.push_back(SecItem{g, g + 1, 0, (uint32_t)((uint64_t)groups[g].la_len * groups[g].lb_len)});
+            else
+                items.push_back(SecItem{g, g1, 0, 0});
+            g = g1;
+        }
+    }
     t->has_map = map_off != nullptr;
     t->nterms = (uint32_t)tz.size();
     t->nentries = (uint32_t)entries.size();
+    t->nitems = (uint32_t)items.size();
 
-    // cost estimate: pair updates + Hamiltonian pair-term evaluations
     uint64_t cost = t->D;
     for (const auto& en : entries) cost += (uint64_t)en.la_len * en.lb_len * 4;
     for (const auto& gr : groups) cost += (uint64_t)gr.la_len * gr.lb_len * (2 + gr.t_count);
     t->cost = cost;
 
-    // device tables: one allocation
+    {  // ansatz tier: the state in LDS when it fits a workgroup's share
+        int dev = 0, max_lds = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) max_lds = 65536;
+        const size_t tab = ((uint64_t)t->Da + t->Db) <= kTableLds ? ((uint64_t)t->Da + t->Db) * 8 : 0;
+        const size_t with_psi = t->D * sizeof(double2) + tab;
+        if (with_psi <= kTier0Lds) {
+            t->tier = 0;
+            t->lds_bytes = with_psi;
+        } else if (with_psi + 1024 <= (size_t)max_lds) {
+            t->tier = 1;
+            t->lds_bytes = with_psi;
+        } else {
+            t->tier = 2;
+            t->lds_bytes = tab;
+        }
+    }
+
     std::vector<uint64_t> af(t->Da), bf(t->Db);
     for (uint32_t a = 0; a < t->Da; ++a) af[a] = spread(sp.str[0][a], 0, norb);
     for (uint32_t b2 = 0; b2 < t->Db; ++b2) bf[b2] = spread(sp.str[1][b2], 1, norb);
-    struct Piece {
-        const void* src;
-        size_t bytes;
-        size_t off;
-    };
-    std::vector<Piece> pieces;
-    size_t total = align256(sizeof(SecTables));
-    auto add = [&](const void* src, size_t bytes) {
-        pieces.push_back({src, bytes, total});
-        total += align256(bytes ? bytes : 1);
-        return pieces.size() - 1;
-    };
-    const size_t i_af = add(af.data(), af.size() * 8), i_bf = add(bf.data(), bf.size() * 8);
-    const size_t i_l = add(lp.pool.data(), lp.pool.size() * sizeof(int2));
-    const size_t i_st = add(steps.data(), steps.size() * sizeof(SecStep));
-    const size_t i_en = add(entries.data(), entries.size() * sizeof(SecEntry));
-    const size_t i_gr = add(groups.data(), groups.size() * sizeof(SecGroup));
-    const size_t i_tz = add(tz.data(), tz.size() * 8);
-    const size_t i_tph = add(tph.data(), tph.size() * sizeof(double2));
-    const size_t i_tc = add(tcoef.data(), tcoef.size() * sizeof(double2));
-    const size_t i_mo = add(moff.data(), moff.size() * 8);
-    const size_t i_mi = add(midx.data(), midx.size() * 4);
-    const size_t i_mv = add(mval.data(), mval.size() * sizeof(double2));
-    QFE_HIP(hipMalloc(&t->dev, total));
+    Packer pk(sizeof(SecTables));
+    const size_t i_af = pk.add(af.data(), af.size() * 8), i_bf = pk.add(bf.data(), bf.size() * 8);
+    const size_t i_as = pk.add(sp.str[0].data(), sp.str[0].size() * 4), i_bs = pk.add(sp.str[1].data(), sp.str[1].size() * 4);
+    const size_t i_ar = pk.add(sp.rank[0].data(), sp.rank[0].size() * 4), i_br = pk.add(sp.rank[1].data(), sp.rank[1].size() * 4);
+    const size_t i_l = pk.add(lp.pool.data(), lp.pool.size() * sizeof(int2));
+    const size_t i_st = pk.add(steps.data(), steps.size() * sizeof(SecStep));
+    const size_t i_en = pk.add(entries.data(), entries.size() * sizeof(SecEntry));
+    const size_t i_gr = pk.add(groups.data(), groups.size() * sizeof(SecGroup));
+    const size_t i_it = pk.add(items.data(), items.size() * sizeof(SecItem));
+    const size_t i_gs = pk.add(gen_step.data(), gen_step.size() * 4);
+    const size_t i_tz = pk.add(tz.data(), tz.size() * 8);
+    const size_t i_tph = pk.add(tph.data(), tph.size() * sizeof(double2));
+    const size_t i_tc = pk.add(tcoef.data(), tcoef.size() * sizeof(double2));
+    const size_t i_mo = pk.add(moff.data(), moff.size() * 8);
+    const size_t i_mi = pk.add(midx.data(), midx.size() * 4);
+    const size_t i_mv = pk.add(mval.data(), mval.size() * sizeof(double2));
+    QFE_HIP(hipMalloc(&t->dev, pk.total));
     char* base = (char*)t->dev;
+    auto at = [&](size_t i) { return base + pk.pieces[i].off; };
     SecTables h{};
     h.D = t->D;
     h.ref_rank = ref_rank;
     h.Da = t->Da;
     h.Db = t->Db;
-    h.afull = (const uint64_t*)(base + pieces[i_af].off);
-    h.bfull = (const uint64_t*)(base + pieces[i_bf].off);
-    h.lists = (const int2*)(base + pieces[i_l].off);
-    h.steps = (const SecStep*)(base + pieces[i_st].off);
-    h.entries = (const SecEntry*)(base + pieces[i_en].off);
-    h.groups = (const SecGroup*)(base + pieces[i_gr].off);
-    h.tz = (const uint64_t*)(base + pieces[i_tz].off);
-    h.tph = (const double2*)(base + pieces[i_tph].off);
-    h.tcoef = (const double2*)(base + pieces[i_tc].off);
-    h.map_off = t->has_map ? (const uint64_t*)(base + pieces[i_mo].off) : nullptr;
-    h.map_idx = (const uint32_t*)(base + pieces[i_mi].off);
-    h.map_val = (const double2*)(base + pieces[i_mv].off);
+    h.afull = (const uint64_t*)at(i_af);
+    h.bfull = (const uint64_t*)at(i_bf);
+    h.astr = (const uint32_t*)at(i_as);
+    h.bstr = (const uint32_t*)at(i_bs);
+    h.arank = (const int32_t*)at(i_ar);
+    h.brank = (const int32_t*)at(i_br);
+    h.lists = (const int2*)at(i_l);
+    h.steps = (const SecStep*)at(i_st);
+    h.entries = (const SecEntry*)at(i_en);
+    h.groups = (const SecGroup*)at(i_gr);
+    h.items = (const SecItem*)at(i_it);
+    h.gen_step = (const uint32_t*)at(i_gs);
+    h.tz = (const uint64_t*)at(i_tz);
+    h.tph = (const double2*)at(i_tph);
+    h.tcoef = (const double2*)at(i_tc);
+    h.map_off = t->has_map ? (const uint64_t*)at(i_mo) : nullptr;
+    h.map_idx = (const uint32_t*)at(i_mi);
+    h.map_val = (const double2*)at(i_mv);
     h.nsteps = (uint32_t)steps.size();
     h.nentries = (uint32_t)entries.size();
     h.ngroups = (uint32_t)groups.size();
     h.nterms = (uint32_t)tz.size();
-    std::vector<char> img(total, 0);
+    h.nitems = (uint32_t)items.size();
+    h.ngen = (uint32_t)ngen;
+    std::vector<char> img;
+    pk.fill(img);
     std::memcpy(img.data(), &h, sizeof(h));
-    for (const auto& p : pieces)
-        if (p.bytes) std::memcpy(img.data() + p.off, p.src, p.bytes);
-    QFE_HIP(hipMemcpy(t->dev, img.data(), total, hipMemcpyHostToDevice));
+    QFE_HIP(hipMemcpy(t->dev, img.data(), pk.total, hipMemcpyHostToDevice));
     t->dt = (SecTables*)t->dev;
     t->afull = h.afull;
     t->bfull = h.bfull;
 
-    // per-request device layout: work buffers (psi, coef, cs) and io (params, inputs)
+    // per-request layout: energy parts first, gradient parts after
     size_t o = 0;
     t->off_psi = o;
     o += align256(t->D * sizeof(double2));
@@ -537,62 +917,109 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     if (t->has_map) o += align256((size_t)t->nterms * sizeof(double2));
     t->off_cs = o;
     o += align256((size_t)t->nentries * sizeof(double2) + 16);
+    t->off_part = o;
+    o += align256((size_t)t->nitems * 8 + 8);
     t->work_bytes = o;
+    t->off_sig = o;
+    o += align256(t->D * sizeof(double2));
+    t->off_gacc = o;
+    o += align256(ngen * 8 + 8);
+    t->grad_bytes = o;
     t->io_count = nparam + ninputs;
-    t->request_bytes = t->work_bytes + align256(t->io_count * 8 + 8) + sizeof(SlotDesc) + 8;
+    t->request_bytes = t->work_bytes + align256(t->io_count * 8 + 8) + sizeof(SlotDesc) + 16;
     return QFE_OK;
 }
 
-SlotDesc make_slot(qfe_template t, char* work, const double* io, double* out, uint32_t mode) {
+SlotDesc make_slot(qfe_template t, char* work, const double* io, double* out, bool grad) {
     SlotDesc d{};
     d.t = t->dt;
     d.psi = (double2*)(work + t->off_psi);
     d.coef = t->has_map ? (double2*)(work + t->off_coef) : nullptr;
     d.cs = (double2*)(work + t->off_cs);
+    d.part = (double*)(work + t->off_part);
+    d.sig = grad ? (double2*)(work + t->off_sig) : nullptr;
+    d.gacc = grad ? (double*)(work + t->off_gacc) : nullptr;
     d.params = io;
     d.inputs = io + t->nparam;
     d.out = out;
-    d.mode = mode;
     return d;
 }
 
-// One request on the library stream: arena = [desc][out][io][work]; the first
-// three parts are uploaded through the pinned staging buffer.
-int run_single(qfe_template t, const double* params, const double* inputs, uint32_t mode, double* energy) {
-    Context& c = ctx();
+// Energy of k requests of one tier: prologue, ansatz, items, sums.
+int launch_energy(const std::vector<qfe_template>& ts, int tier, size_t lds, const SlotDesc* ddesc, const uint32_t* dprefix,
+                  uint32_t total_items, hipStream_t st) {
+    const unsigned k = (unsigned)ts.size();
+    hipLaunchKernelGGL(k_sec_prologue, dim3(k), dim3(256), 0, st, ddesc, 1);
+    QFE_HIP(hipGetLastError());
+    int rc = launch_ansatz(tier, k, lds, st, ddesc);
+    if (rc) return rc;
+    if (total_items) {
+        hipLaunchKernelGGL(k_sec_energy, dim3(total_items), dim3(kItemThreads), 0, st, ddesc, dprefix, (uint32_t)k);
+        QFE_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_sec_sum, dim3(k), dim3(256), 0, st, ddesc);
+    QFE_HIP(hipGetLastError());
+    return QFE_OK;
+}
+
+// Single request on the library stream.  Arena: [desc 256][out 256][prefix 256][io][work + gradient parts].
+struct Single {
+    SlotDesc* ddesc;
+    double* dout;
+    uint32_t* dprefix;
+    char* work;
+};
+
+int single_setup(qfe_template t, const double* params, const double* inputs, bool grad, Single& S) {
     const size_t io_bytes = align256(t->io_count * 8 + 8);
-    const size_t head = 512 + io_bytes;
-    if (!t->arena) QFE_HIP(hipMalloc(&t->arena, head + t->work_bytes));
+    const size_t head = 768 + io_bytes;
+    if (!t->arena) QFE_HIP(hipMalloc(&t->arena, head + t->grad_bytes));
     char* dev = (char*)t->arena;
-    double* dout = (double*)(dev + 256);
-    const double* dio = (const double*)(dev + 512);
+    S.ddesc = (SlotDesc*)dev;
+    S.dout = (double*)(dev + 256);
+    S.dprefix = (uint32_t*)(dev + 512);
+    S.work = dev + head;
     void* host = nullptr;
     int rc = staging_begin(head, &host);
     if (rc) return rc;
     char* h = (char*)host;
-    SlotDesc d = make_slot(t, dev + head, dio, dout, mode);
+    SlotDesc d = make_slot(t, S.work, (const double*)(dev + 768), S.dout, grad);
     std::memcpy(h, &d, sizeof(d));
-    if (t->nparam) std::memcpy(h + 512, params, t->nparam * 8);
-    if (t->ninputs) std::memcpy(h + 512 + t->nparam * 8, inputs, t->ninputs * 8);
-    rc = staging_commit(dev, head);
+    const uint32_t pre[2] = {0, t->nitems};
+    std::memcpy(h + 512, pre, sizeof(pre));
+    if (t->nparam) std::memcpy(h + 768, params, t->nparam * 8);
+    if (t->ninputs) std::memcpy(h + 768 + t->nparam * 8, inputs, t->ninputs * 8);
+    return staging_commit(dev, head);
+}
+
+int fetch_doubles(const double* dev, size_t count, double* host) {
+    void* pin = nullptr;
+    int rc = pinned_reserve(count * 8 + 8, &pin);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sector, dim3(1), dim3(kSecThreads), 0, c.stream, (const SlotDesc*)dev);
+    QFE_HIP(hipMemcpyAsync(pin, dev, count * 8, hipMemcpyDeviceToHost, ctx().stream));
+    QFE_HIP(hipStreamSynchronize(ctx().stream));
+    std::memcpy(host, pin, count * 8);
+    return QFE_OK;
+}
+
+// psi = U|ref> and sigma = H psi for one request (gradient paths).
+int single_psi_sigma(qfe_template t, const double* params, const double* inputs, Single& S) {
+    int rc = single_setup(t, params, inputs, true, S);
+    if (rc) return rc;
+    hipStream_t st = ctx().stream;
+    hipLaunchKernelGGL(k_sec_prologue, dim3(1), dim3(256), 0, st, S.ddesc, 1);
     QFE_HIP(hipGetLastError());
-    if (mode == 0) {
-        void* pin = nullptr;
-        rc = pinned_reserve(8, &pin);
-        if (rc) return rc;
-        QFE_HIP(hipMemcpyAsync(pin, dout, 8, hipMemcpyDeviceToHost, c.stream));
-        QFE_HIP(hipStreamSynchronize(c.stream));
-        *energy = *(double*)pin;
-    }
+    rc = launch_ansatz(t->tier, 1, t->lds_bytes, st, S.ddesc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sec_hpsi, dim3((unsigned)((t->D + 255) / 256), 1), dim3(256), 0, st, S.ddesc);
+    QFE_HIP(hipGetLastError());
     return QFE_OK;
 }
 
 }  // namespace
 }  // namespace qfe
 
-// ---- serving engine -------------------------------------------------------------------
+// ---- serving engine -----------------------------------------------------------------------
 
 namespace {
 
@@ -613,6 +1040,8 @@ struct Lane {
     size_t out_cap = 0;
     std::vector<uint64_t> ids;  // in slot order
     bool busy = false;
+    int tier = 0;
+    uint64_t seq = 0;  // launch order
 };
 
 }  // namespace
@@ -622,118 +1051,156 @@ struct qfe_batch_s {
     int max_active = 0;
     uint64_t next_id = 1;
     uint64_t active = 0;
-    std::deque<Request> queue;
+    uint64_t seq = 0;
+    std::deque<Request> queue[3];  // one queue per ansatz tier
     std::vector<Lane> lanes;
-    std::deque<int> inflight;
     std::deque<std::pair<uint64_t, double>> done;
 };
 
 namespace {
 
-constexpr int kLanes = 3;
+// Requests of different tiers never share a launch, so small requests do not
+// wait for the longest request of a mixed launch.
+constexpr int kLanesPerTier = 2;
+constexpr int kLanes = 3 * kLanesPerTier;
+
+template <typename T>
+int grow_device(void*& p, size_t& cap, size_t need) {
+    if (cap >= need) return QFE_OK;
+    if (p) QFE_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    QFE_HIP(hipMalloc(&p, need + need / 4));
+    cap = need + need / 4;
+    return QFE_OK;
+}
+
+int grow_pinned(void*& p, size_t& cap, size_t need) {
+    if (cap >= need) return QFE_OK;
+    if (p) QFE_HIP(hipHostFree(p));
+    p = nullptr;
+    cap = 0;
+    QFE_HIP(hipHostMalloc(&p, need + need / 4, hipHostMallocDefault));
+    cap = need + need / 4;
+    return QFE_OK;
+}
 
 int lane_launch(qfe_batch b, Lane& L) {
-    // admit from the front of the queue while the footprint fits this lane's share
+    // admit from the front of the tier's queue while the footprint fits this lane's share
     const uint64_t share = b->budget / kLanes;
+    std::deque<Request>& queue = b->queue[L.tier];
     std::vector<Request> take;
     uint64_t foot = 0;
-    while (!b->queue.empty() && (int64_t)(b->active + take.size()) < b->max_active) {
-        const Request& r = b->queue.front();
-        const uint64_t f = r.t->request_bytes;
+    while (!queue.empty() && (int64_t)(b->active + take.size()) < b->max_active) {
+        const uint64_t f = queue.front().t->request_bytes;
         if (!take.empty() && foot + f > share) break;
         foot += f;
-        take.push_back(std::move(b->queue.front()));
-        b->queue.pop_front();
+        take.push_back(std::move(queue.front()));
+        queue.pop_front();
     }
     if (take.empty()) return QFE_OK;
     // heaviest requests first, so their workgroups start first
     std::stable_sort(take.begin(), take.end(), [](const Request& x, const Request& y) { return x.t->cost > y.t->cost; });
     const size_t k = take.size();
-    size_t io_total = 0, work_total = 0;
-    for (const auto& r : take) {
-        io_total += r.t->io_count;
-        work_total += r.t->work_bytes;
+    size_t io_total = 0, work_total = 0, lds = 0;
+    uint64_t items = 0;
+    std::vector<qfe_template> ts(k);
+    for (size_t i = 0; i < k; ++i) {
+        ts[i] = take[i].t;
+        io_total += ts[i]->io_count;
+        work_total += ts[i]->work_bytes;
+        lds = std::max(lds, ts[i]->lds_bytes);
+        items += ts[i]->nitems;
     }
-    const size_t head = align256(k * sizeof(SlotDesc)) + align256(k * 8) + align256(io_total * 8 + 8);
+    if (items >= (1ull << 31)) return fail(QFE_ERR_ARG, "qfe_batch: too many work items in one launch");
+    const size_t o_out = align256(k * sizeof(SlotDesc));
+    const size_t o_pre = o_out + align256(k * 8);
+    const size_t o_io = o_pre + align256((k + 1) * 4);
+    const size_t head = o_io + align256(io_total * 8 + 8);
     const size_t need = head + work_total;
-    if (L.dev_cap < need) {
-        if (L.dev) QFE_HIP(hipFree(L.dev));
-        L.dev = nullptr;
-        L.dev_cap = 0;
-        QFE_HIP(hipMalloc(&L.dev, need + need / 4));
-        L.dev_cap = need + need / 4;
+    int rc = grow_device<char>(L.dev, L.dev_cap, need);
+    if (!rc) rc = grow_pinned(L.up, L.up_cap, head);
+    if (!rc) {
+        void* p = L.out;
+        size_t cap = L.out_cap * 8;
+        rc = grow_pinned(p, cap, k * 8 + 8);
+        L.out = (double*)p;
+        L.out_cap = cap / 8;
     }
-    if (L.up_cap < head) {
-        if (L.up) QFE_HIP(hipHostFree(L.up));
-        L.up = nullptr;
-        L.up_cap = 0;
-        QFE_HIP(hipHostMalloc(&L.up, head + head / 4, hipHostMallocDefault));
-        L.up_cap = head + head / 4;
-    }
-    if (L.out_cap < k) {
-        if (L.out) QFE_HIP(hipHostFree(L.out));
-        L.out = nullptr;
-        L.out_cap = 0;
-        QFE_HIP(hipHostMalloc((void**)&L.out, (k + 64) * 8, hipHostMallocDefault));
-        L.out_cap = k + 64;
-    }
+    if (rc) return rc;
     char* dev = (char*)L.dev;
     char* up = (char*)L.up;
-    const size_t o_out = align256(k * sizeof(SlotDesc)), o_io = o_out + align256(k * 8);
     double* dout = (double*)(dev + o_out);
+    uint32_t* hpre = (uint32_t*)(up + o_pre);
     size_t io_pos = 0, work_pos = head;
+    uint32_t acc_items = 0;
     L.ids.clear();
     for (size_t i = 0; i < k; ++i) {
-        qfe_template t = take[i].t;
+        qfe_template t = ts[i];
         const double* dio = (const double*)(dev + o_io) + io_pos;
         double* hio = (double*)(up + o_io) + io_pos;
-        SlotDesc d = make_slot(t, dev + work_pos, dio, dout + i, 0);
+        SlotDesc d = make_slot(t, dev + work_pos, dio, dout + i, false);
         std::memcpy(up + i * sizeof(SlotDesc), &d, sizeof(d));
         if (t->nparam) std::memcpy(hio, take[i].params.data(), t->nparam * 8);
         if (t->ninputs) std::memcpy(hio + t->nparam, take[i].inputs.data(), t->ninputs * 8);
+        hpre[i] = acc_items;
+        acc_items += t->nitems;
         io_pos += t->io_count;
         work_pos += t->work_bytes;
         L.ids.push_back(take[i].id);
     }
+    hpre[k] = acc_items;
     QFE_HIP(hipMemcpyAsync(dev, up, head, hipMemcpyHostToDevice, L.stream));
-    hipLaunchKernelGGL(k_sector, dim3((unsigned)k), dim3(kSecThreads), 0, L.stream, (const SlotDesc*)dev);
-    QFE_HIP(hipGetLastError());
+    rc = launch_energy(ts, L.tier, lds, (const SlotDesc*)dev, (const uint32_t*)(dev + o_pre), acc_items, L.stream);
+    if (rc) return rc;
     QFE_HIP(hipMemcpyAsync(L.out, dout, k * 8, hipMemcpyDeviceToHost, L.stream));
     QFE_HIP(hipEventRecord(L.done, L.stream));
     L.busy = true;
+    L.seq = ++b->seq;
     b->active += k;
     return QFE_OK;
 }
 
+void retire(qfe_batch b, Lane& L) {
+    for (size_t i = 0; i < L.ids.size(); ++i) b->done.emplace_back(L.ids[i], L.out[i]);
+    b->active -= L.ids.size();
+    L.ids.clear();
+    L.busy = false;
+}
+
+// Retire every finished launch; with wait_oldest, first block on the oldest
+// launch still running.
 int collect(qfe_batch b, bool wait_oldest) {
-    bool waited = !wait_oldest;
-    while (!b->inflight.empty()) {
-        Lane& L = b->lanes[b->inflight.front()];
-        hipError_t q;
-        if (!waited) {
-            q = hipEventSynchronize(L.done);
-            waited = true;
-        } else {
-            q = hipEventQuery(L.done);
+    if (wait_oldest) {
+        Lane* oldest = nullptr;
+        for (auto& L : b->lanes)
+            if (L.busy && (!oldest || L.seq < oldest->seq)) oldest = &L;
+        if (oldest) {
+            QFE_HIP(hipEventSynchronize(oldest->done));
+            retire(b, *oldest);
         }
-        if (q == hipErrorNotReady) break;
+    }
+    for (auto& L : b->lanes) {
+        if (!L.busy) continue;
+        const hipError_t q = hipEventQuery(L.done);
+        if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) return hip_fail(q, "qfe_batch: launch completion");
-        for (size_t i = 0; i < L.ids.size(); ++i) b->done.emplace_back(L.ids[i], L.out[i]);
-        b->active -= L.ids.size();
-        L.ids.clear();
-        L.busy = false;
-        b->inflight.pop_front();
+        retire(b, L);
     }
     return QFE_OK;
 }
 
+bool any_busy(qfe_batch b) {
+    for (const auto& L : b->lanes)
+        if (L.busy) return true;
+    return false;
+}
+
 int fill_lanes(qfe_batch b) {
-    for (int i = 0; i < (int)b->lanes.size() && !b->queue.empty(); ++i) {
-        Lane& L = b->lanes[i];
-        if (L.busy) continue;
+    for (auto& L : b->lanes) {
+        if (L.busy || b->queue[L.tier].empty()) continue;
         int rc = lane_launch(b, L);
         if (rc) return rc;
-        if (L.busy) b->inflight.push_back(i);
     }
     return QFE_OK;
 }
@@ -795,23 +1262,105 @@ int qfe_template_info(qfe_template t, uint64_t* sector_dim, uint64_t* request_by
 int qfe_template_energy(qfe_template t, const double* params, const double* inputs, double* energy) {
     if (!t || !energy || (t->nparam && !params) || (t->ninputs && !inputs))
         return fail(QFE_ERR_ARG, "qfe_template_energy: null argument");
-    return run_single(t, params, inputs, 0, energy);
+    Single S;
+    int rc = single_setup(t, params, inputs, false, S);
+    if (rc) return rc;
+    rc = launch_energy({t}, t->tier, t->lds_bytes, S.ddesc, S.dprefix, t->nitems, ctx().stream);
+    if (rc) return rc;
+    return fetch_doubles(S.dout, 1, energy);
 }
 
 int qfe_template_prepare(qfe_template t, const double* params, qfe_state out) {
     if (!t || !out || (t->nparam && !params)) return fail(QFE_ERR_ARG, "qfe_template_prepare: null argument");
     if (out->n != t->n) return fail(QFE_ERR_ARG, "qfe_template_prepare: state has the wrong number of qubits");
     std::vector<double> zeros(t->ninputs + 1, 0.0);
-    int rc = run_single(t, params, zeros.data(), 1, nullptr);
+    Single S;
+    int rc = single_setup(t, params, zeros.data(), false, S);
     if (rc) return rc;
-    rc = launch_zero(out->d, out->N, ctx().stream);
+    hipStream_t st = ctx().stream;
+    hipLaunchKernelGGL(k_sec_prologue, dim3(1), dim3(256), 0, st, S.ddesc, 0);
+    QFE_HIP(hipGetLastError());
+    rc = launch_ansatz(t->tier, 1, t->lds_bytes, st, S.ddesc);
+    if (!rc) rc = launch_zero(out->d, out->N, st);
     if (rc) return rc;
-    const char* work = (const char*)t->arena + 512 + align256(t->io_count * 8 + 8);
-    const unsigned grid = (unsigned)((t->D + 255) / 256);
-    hipLaunchKernelGGL(k_sector_scatter, dim3(grid), dim3(256), 0, ctx().stream, (const double2*)(work + t->off_psi), t->D,
-                       t->Db, t->afull, t->bfull, out->d);
+    hipLaunchKernelGGL(k_sector_scatter, dim3((unsigned)((t->D + 255) / 256)), dim3(256), 0, st,
+                       (const double2*)(S.work + t->off_psi), t->D, t->Db, t->afull, t->bfull, out->d);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
+}
+
+int qfe_template_gradient(qfe_template t, const double* params, const double* inputs, double* grad, double* energy) {
+    if (!t || !grad || (t->nparam && !params) || (t->ninputs && !inputs))
+        return fail(QFE_ERR_ARG, "qfe_template_gradient: null argument");
+    Single S;
+    int rc = single_psi_sigma(t, params, inputs, S);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_sec_reverse<1024>), dim3(1), dim3(1024), 0, ctx().stream, S.ddesc);
+    QFE_HIP(hipGetLastError());
+    std::vector<double> acc(t->ngen + 1);
+    rc = fetch_doubles((const double*)(S.work + t->off_gacc), t->ngen, acc.data());
+    if (!rc) rc = fetch_doubles(S.dout, 1, &acc[t->ngen]);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < t->nparam; ++i) grad[i] = 0.0;
+    for (uint64_t k = 0; k < t->ngen; ++k) grad[t->gparam[k]] += 2.0 * t->gscale[k] * acc[k];
+    if (energy) *energy = acc[t->ngen];
+    return QFE_OK;
+}
+
+int qfe_template_pool_gradients(qfe_template t, const double* params, const double* inputs, const qfe_pauli* pool_terms,
+                                const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads) {
+    if (!t || (nops && (!pool_terms || !op_offsets || !grads)) || (t->nparam && !params) || (t->ninputs && !inputs))
+        return fail(QFE_ERR_ARG, "qfe_template_pool_gradients: null argument");
+    if (batch == 0) batch = 64;
+    // pool operators as pair rotations on the template's sector
+    ListPool lp;
+    std::vector<SecStep> steps;
+    std::vector<SecEntry> entries;
+    std::vector<PoolOp> ops(nops);
+    const uint64_t lim = (1ull << t->n) - 1ull;
+    for (uint64_t o = 0; o < nops; ++o) {
+        if (op_offsets[o + 1] < op_offsets[o]) return fail(QFE_ERR_ARG, "qfe_template_pool_gradients: offsets must not decrease");
+        ops[o].e0 = (uint32_t)entries.size();
+        int rc = build_runs(t->sp, lp, t->norb, lim, pool_terms, op_offsets[o], op_offsets[o + 1], 0, 1.0, (uint32_t)o, steps,
+                            entries, "qfe_template_pool_gradients");
+        if (rc) return rc;
+        ops[o].e1 = (uint32_t)entries.size();
+    }
+    Single S;
+    int rc = single_psi_sigma(t, params, inputs, S);
+    if (rc) return rc;
+    Packer pk(0);
+    const size_t i_l = pk.add(lp.pool.data(), lp.pool.size() * sizeof(int2));
+    const size_t i_e = pk.add(entries.data(), entries.size() * sizeof(SecEntry));
+    const size_t i_o = pk.add(ops.data(), ops.size() * sizeof(PoolOp));
+    const size_t o_out = pk.total;
+    const size_t total = o_out + align256(nops * 8 + 8);
+    void* dev = nullptr;
+    QFE_HIP(hipMalloc(&dev, total));
+    std::vector<char> img;
+    pk.fill(img);
+    hipError_t e = hipMemcpy(dev, img.data(), pk.total, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(dev);
+        return hip_fail(e, "qfe_template_pool_gradients: upload");
+    }
+    char* base = (char*)dev;
+    const double2* psi = (const double2*)(S.work + t->off_psi);
+    const double2* sig = (const double2*)(S.work + t->off_sig);
+    // batches of `batch` operators per launch
+    for (uint64_t b0 = 0; b0 < nops; b0 += batch) {
+        const uint64_t b1 = std::min(nops, b0 + batch);
+        hipLaunchKernelGGL(k_sec_pool, dim3((unsigned)(b1 - b0)), dim3(256), 0, ctx().stream, t->dt,
+                           (const SecEntry*)(base + pk.pieces[i_e].off), (const int2*)(base + pk.pieces[i_l].off),
+                           (const PoolOp*)(base + pk.pieces[i_o].off) + b0, sig, psi, (double*)(base + o_out) + b0);
+        e = hipGetLastError();
+        if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess && nops) rc = fetch_doubles((const double*)(base + o_out), nops, grads);
+    (void)hipStreamSynchronize(ctx().stream);
+    (void)hipFree(dev);
+    if (e != hipSuccess) return hip_fail(e, "qfe_template_pool_gradients: launch");
+    return rc;
 }
 
 int qfe_batch_create(uint64_t hbm_budget_bytes, int max_active, qfe_batch* out) {
@@ -822,6 +1371,7 @@ int qfe_batch_create(uint64_t hbm_budget_bytes, int max_active, qfe_batch* out) 
     b->budget = hbm_budget_bytes;
     b->max_active = max_active;
     b->lanes.resize(kLanes);
+    for (int i = 0; i < kLanes; ++i) b->lanes[i].tier = i / kLanesPerTier;
     for (auto& L : b->lanes) {
         hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
@@ -858,8 +1408,8 @@ int qfe_batch_submit(qfe_batch b, qfe_template t, const double* params, const do
     r.t = t;
     r.params.assign(params, params + t->nparam);
     if (t->ninputs) r.inputs.assign(inputs, inputs + t->ninputs);
-    b->queue.push_back(std::move(r));
-    *request_id = b->queue.back().id;
+    *request_id = r.id;
+    b->queue[t->tier].push_back(std::move(r));
     return QFE_OK;
 }
 
@@ -875,7 +1425,7 @@ int qfe_batch_step(qfe_batch b, uint64_t* done_ids, double* done_energies, uint6
     if (!b || !ndone || (cap && (!done_ids || !done_energies))) return fail(QFE_ERR_ARG, "qfe_batch_step: invalid argument");
     int rc = collect(b, false);
     if (!rc) rc = fill_lanes(b);
-    if (!rc && b->done.empty() && !b->inflight.empty()) {
+    if (!rc && b->done.empty() && any_busy(b)) {
         rc = collect(b, true);
         if (!rc) rc = fill_lanes(b);
     }
@@ -885,7 +1435,7 @@ int qfe_batch_step(qfe_batch b, uint64_t* done_ids, double* done_energies, uint6
 
 int qfe_batch_pending(qfe_batch b, uint64_t* queued, uint64_t* active) {
     if (!b) return fail(QFE_ERR_ARG, "qfe_batch_pending: invalid handle");
-    if (queued) *queued = b->queue.size();
+    if (queued) *queued = b->queue[0].size() + b->queue[1].size() + b->queue[2].size();
     if (active) *active = b->active + b->done.size();
     return QFE_OK;
 }

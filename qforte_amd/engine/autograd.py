@@ -1,16 +1,36 @@
 """UCC energy as a differentiable torch function.
 
-Forward: E(t) from the compiled program (one sweep of exponentials and one
-grouped expectation pass per X mask).  Backward: the hand-written adjoint
-gradient (``qfe_program_gradient``): sigma = H psi, then a reverse sweep that
-un-applies each exponential on psi and sigma with one extra state, giving
-dE/dt_k = 2 Re <sigma_k| K_k |psi_k> -- the recursion of the reference's
-``UCCVQE.measure_gradient`` (src/qforte/abc/uccvqeabc.py:122-251).
-PyTorch is used only for this binding; all arithmetic runs in the engine.
+Forward: E(t) from a compiled program.  Backward: the hand-written adjoint
+gradient: sigma = H psi, then a reverse sweep that un-applies each generator
+on psi and sigma, giving dE/dt_k = 2 Re <sigma_k| K_k |psi_k> -- the
+recursion of the reference's ``UCCVQE.measure_gradient``
+(src/qforte/abc/uccvqeabc.py:122-251).  The program is a full-space
+``UCCProgram`` (``qfe_program_gradient``) or a ``SectorProgram`` over a
+determinant-sector template (``qfe_template_gradient``).  PyTorch is used only
+for this binding; all arithmetic runs in the engine.
 """
 
 import numpy as np
 import torch
+
+
+class SectorProgram:
+    """A ``SectorTemplate`` bound to one request's Hamiltonian inputs, with
+    the ``energy(t)`` / ``gradient(t)`` interface of ``UCCProgram``."""
+
+    def __init__(self, template, inputs=()):
+        self.template = template
+        self.inputs = [float(v) for v in inputs]
+
+    @property
+    def nparam(self):
+        return self.template.nparam
+
+    def energy(self, t):
+        return self.template.energy(list(t), self.inputs)
+
+    def gradient(self, t):
+        return self.template.gradient(list(t), self.inputs)
 
 
 class UCCEnergy(torch.autograd.Function):

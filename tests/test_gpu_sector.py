@@ -138,3 +138,58 @@ def test_serving_respects_the_budget(qf):
     with pytest.raises(Exception, match="budget"):
         small = qf.ServingEngine(hbm_budget_bytes=tpl.request_bytes, max_active=8)
         small.submit(tpl, t, inputs)
+
+
+def test_sector_gradient_matches_program_and_finite_differences(qf):
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    n = len(mol.hf_reference)
+    pool = sd_pool(qf, mol.hf_reference)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    tpl = qf.compile_ucc_template(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    t = np.random.default_rng(4).normal(0.0, 0.1, len(tops))
+    e, g = tpl.gradient(list(t))
+    e_p, g_p = prog.gradient(list(t))
+    assert abs(e - e_p) < E_TOL
+    assert np.max(np.abs(np.array(g) - np.array(g_p))) < 1e-10
+    h = 1e-5
+    for k in range(len(tops)):
+        tp, tm = t.copy(), t.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (tpl.energy(list(tp)) - tpl.energy(list(tm))) / (2 * h)
+        assert abs(fd - g[k]) < 1e-8, (k, fd, g[k])
+
+
+def test_sector_gradient_with_integral_map(qf):
+    norb, na, nb = 5, 3, 2
+    h, g, ref = synthetic_case(qf, norb, na, nb, 21)
+    H = qf.build_sq_hamiltonian(h, g, 0.0).jw_transform()
+    pool = sd_pool(qf, ref)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(2 * norb, ref, H, pool, tops)
+    hmap = qf.HamiltonianMap(norb)
+    tpl = qf.compile_ucc_template(2 * norb, ref, hmap, pool, tops)
+    t = list(np.random.default_rng(6).normal(0.0, 0.05, len(tops)))
+    e, gr = tpl.gradient(t, hmap.pack(0.0, h, g))
+    e_p, g_p = prog.gradient(t)
+    assert abs(e - e_p) < E_TOL
+    assert np.max(np.abs(np.array(gr) - np.array(g_p))) < 1e-10
+
+
+def test_sector_pool_gradients_match_program(qf):
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    n = len(mol.hf_reference)
+    pool = sd_pool(qf, mol.hf_reference)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    tpl = qf.compile_ucc_template(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    gsd = qf.SQOpPool()
+    gsd.set_orb_spaces(mol.hf_reference)
+    gsd.fill_pool("GSD")
+    ops, _, _ = qf.ucc_generators(gsd, list(range(len(gsd))))
+    t = list(np.random.default_rng(8).normal(0.0, 0.1, len(tops)))
+    a = np.array(tpl.pool_gradients(t, ops, 7))
+    b = np.array(prog.pool_gradients(t, ops, 64))
+    assert a.shape == b.shape
+    assert np.max(np.abs(a - b)) < 1e-10
