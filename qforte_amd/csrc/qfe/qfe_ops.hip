@@ -594,6 +594,86 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
     for (int i = 0; i < (1 << M); ++i) s[off[i]] = a[i];
 }
 
+// ---------------------------------------------------------------------------
+// Device: span tile for longer runs of Pauli exponentials
+//
+// The flip masks of a run span D dimensions with a reduced basis b_0..b_{D-1}
+// whose pivots (highest bits) all lie at or above L.  A tile is fixed by the
+// bits of the index outside the pivots and outside the low L bits (the block
+// base J); it holds the 2^(L+D) amplitudes (J | l) ^ c_i, l < 2^L, c_i = XOR
+// of b_k over the bits k of i, in LDS as tile[i * 2^L + l].  Exponential o
+// with flip x = c_v pairs (i, l) with (i ^ v, l), and its sign
+// (-1)^{idx.z} splits into parity(J.z) ^ parity(l.z) ^ parity(i.zb), zb bit k
+// = parity(b_k.z).  Each tile is read and written once per pass; up to
+// kSpanK - L exponentials with independent masks share the pass, against 4 in
+// the register orbit.
+// ---------------------------------------------------------------------------
+
+constexpr int kSpanK = 12;  // tile of at most 2^kSpanK amplitudes (64 KiB of LDS)
+constexpr int kSpanTPB = 256;
+
+struct SpanOp {
+    uint32_t v;   // flip in tile coordinates (0: diagonal exponential)
+    uint32_t zb;  // bit k = parity(b_k & z)
+    uint32_t hv;  // highest bit of v (pairs enumerate i with it clear)
+    uint32_t pad;
+    uint64_t z;
+    double c, sn;
+    double2 ph;
+};
+
+struct SpanDesc {
+    uint32_t L, D, nops, pad;
+    uint32_t piv[kSpanK];  // pivot bits, ascending
+};
+
+__device__ __forceinline__ uint32_t ins0_32(uint32_t k, uint32_t h) {
+    const uint32_t lo = k & ((1u << h) - 1u);
+    return ((k ^ lo) << 1) | lo;
+}
+
+__global__ __launch_bounds__(kSpanTPB) void k_span(double2* __restrict__ s, const SpanDesc* __restrict__ pdesc,
+                                                   const uint64_t* __restrict__ ctab, const SpanOp* __restrict__ ops) {
+    extern __shared__ double2 tile[];
+    const uint32_t L = pdesc->L, D = pdesc->D, nops = pdesc->nops;
+    const uint32_t lmask = (1u << L) - 1u;
+    uint64_t J = (uint64_t)blockIdx.x << L;
+    for (uint32_t k = 0; k < D; ++k) J = ins0(J, pdesc->piv[k]);
+    const uint32_t size = 1u << (L + D);
+    for (uint32_t e = threadIdx.x; e < size; e += kSpanTPB) tile[e] = s[(J | (e & lmask)) ^ ctab[e >> L]];
+    __syncthreads();
+    for (uint32_t o = 0; o < nops; ++o) {
+        const SpanOp op = ops[o];
+        const uint32_t pj = (uint32_t)__popcll(J & op.z) & 1u;
+        const uint32_t zl = (uint32_t)op.z & lmask;
+        const double2 iph = make_double2(-op.ph.y, op.ph.x);  // i * phase
+        if (op.v == 0) {
+            for (uint32_t e = threadIdx.x; e < size; e += kSpanTPB) {
+                const uint32_t sg = pj ^ ((uint32_t)__popc((e & lmask) & zl) & 1u) ^ ((uint32_t)__popc((e >> L) & op.zb) & 1u);
+                const double sn = sg ? -op.sn : op.sn;
+                tile[e] = cmul(make_double2(op.c + sn * iph.x, sn * iph.y), tile[e]);
+            }
+        } else {
+            const uint32_t half = size >> 1;
+            for (uint32_t q = threadIdx.x; q < half; q += kSpanTPB) {
+                const uint32_t l = q & lmask;
+                const uint32_t i1 = ins0_32(q >> L, op.hv), i2 = i1 ^ op.v;
+                const uint32_t e1 = (i1 << L) | l, e2 = (i2 << L) | l;
+                const uint32_t pl = pj ^ ((uint32_t)__popc(l & zl) & 1u);
+                const double s1 = (pl ^ ((uint32_t)__popc(i1 & op.zb) & 1u)) ? -op.sn : op.sn;
+                const double s2 = (pl ^ ((uint32_t)__popc(i2 & op.zb) & 1u)) ? -op.sn : op.sn;
+                const double2 a = tile[e1], b = tile[e2];
+                const double2 ta = cmul(iph, a), tb = cmul(iph, b);
+                // a' = c a + i sin ph (-1)^{idx2.z} b ; b' = c b + i sin ph (-1)^{idx1.z} a
+                tile[e1] = make_double2(op.c * a.x + s2 * tb.x, op.c * a.y + s2 * tb.y);
+                tile[e2] = make_double2(op.c * b.x + s1 * ta.x, op.c * b.y + s1 * ta.y);
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < size; e += kSpanTPB) s[(J | (e & lmask)) ^ ctab[e >> L]] = tile[e];
+}
+
 // Reduced GF(2) basis with pivot = highest bit; every pivot bit appears in
 // exactly one basis vector.
 struct Span {
@@ -603,11 +683,12 @@ struct Span {
             if ((x >> (63 - __builtin_clzll(v))) & 1ull) x ^= v;
         return x;
     }
-    bool add(uint64_t x, int maxdim) {
+    bool add(uint64_t x, int maxdim, int minpiv = 0) {
         const uint64_t r = reduce(x);
         if (!r) return true;
         if ((int)b.size() >= maxdim) return false;
         const int p = 63 - __builtin_clzll(r);
+        if (p < minpiv) return false;
         for (uint64_t& v : b)
             if ((v >> p) & 1ull) v ^= r;
         b.push_back(r);
@@ -615,13 +696,14 @@ struct Span {
     }
 };
 
-enum PassKind { P_SINGLE = 0, P_TILE = 1, P_ORBIT = 2 };
+enum PassKind { P_SINGLE = 0, P_TILE = 1, P_ORBIT = 2, P_SPAN = 3 };
 
 struct Pass {
     int kind = P_SINGLE;
     uint64_t bmask = 0;
     std::vector<int> ops;  // indices into the lowered op list
-    Span span;             // P_ORBIT
+    Span span;             // P_ORBIT, P_SPAN
+    int span_L = 0;        // P_SPAN: contiguous low bits of a tile
 };
 
 // Greedy planner.  Ops may move ahead of an earlier op only when their qubit
@@ -655,9 +737,9 @@ static void grow_tile(const std::vector<HostOp>& ops, const std::vector<char>& d
 }
 
 static void grow_orbit(const std::vector<HostOp>& ops, const std::vector<char>& done, size_t first, uint64_t all,
-                       int max_ops, Pass& p) {
+                       int max_ops, Pass& p, int kind = P_ORBIT, int maxdim = kOrbitMax, int minpiv = 0) {
     p = Pass{};
-    p.kind = P_ORBIT;
+    p.kind = kind;
     uint64_t blocked = 0;
     for (size_t i = first; i < ops.size(); ++i) {
         if (done[i]) continue;
@@ -668,7 +750,7 @@ static void grow_orbit(const std::vector<HostOp>& ops, const std::vector<char>& 
             continue;
         }
         Span trial = p.span;
-        if (h.op.kind == K_PEXP && (int)p.ops.size() < max_ops && trial.add(h.op.x, kOrbitMax)) {
+        if (h.op.kind == K_PEXP && (int)p.ops.size() < max_ops && trial.add(h.op.x, maxdim, minpiv)) {
             p.span = trial;
             p.ops.push_back((int)i);
         } else {
@@ -688,11 +770,19 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
     while (first < ops.size()) {
         while (first < ops.size() && done[first]) ++first;
         if (first >= ops.size()) break;
-        Pass tile, orbit;
+        Pass tile, orbit, span;
         grow_tile(ops, done, first, K, lowmask, all, tu.max_ops, tile);
         grow_orbit(ops, done, first, all, tu.max_ops, orbit);
+        // span tiles: low L bits contiguous, up to kSpanK - L dimensions, and
+        // at least 2^8 tiles so the grid fills the chip
+        const int SL = std::min(tu.low_qubits, 6);
+        const int SD = std::min(kSpanK - SL, n - SL - 8);
+        if (SD > kOrbitMax) grow_orbit(ops, done, first, all, tu.max_ops, span, P_SPAN, SD, SL);
+        span.span_L = SL;
         Pass p;
-        if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
+        if (span.ops.size() > orbit.ops.size() && span.ops.size() >= tile.ops.size() && span.span.b.size() > (size_t)kOrbitMax) {
+            p = std::move(span);
+        } else if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
             p = std::move(orbit);
         } else if (tile.ops.size() >= 2) {
             p = std::move(tile);
@@ -765,6 +855,57 @@ static void build_orbit(const Pass& p, const std::vector<HostOp>& ops, OrbitDesc
         o.ph = g.m[1];
         out.push_back(o);
     }
+}
+
+// Span descriptor, orbit offsets and per-op data for a finished span pass.
+static void build_span(const Pass& p, const std::vector<HostOp>& ops, SpanDesc& d, std::vector<uint64_t>& ctab,
+                       std::vector<SpanOp>& out) {
+    std::vector<uint64_t> b = p.span.b;
+    std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
+    const int D = (int)b.size();
+    d = SpanDesc{};
+    d.L = (uint32_t)p.span_L;
+    d.D = (uint32_t)D;
+    d.nops = (uint32_t)p.ops.size();
+    for (int k = 0; k < D; ++k) d.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+    for (int i = 0; i < (1 << D); ++i) {
+        uint64_t c = 0;
+        for (int k = 0; k < D; ++k)
+            if ((i >> k) & 1) c ^= b[k];
+        ctab.push_back(c);
+    }
+    for (int oi : p.ops) {
+        const DevOp& g = ops[oi].op;
+        SpanOp o{};
+        for (int k = 0; k < D; ++k) {
+            if ((g.x >> d.piv[k]) & 1ull) o.v |= 1u << k;
+            if (__builtin_popcountll(b[k] & g.z) & 1) o.zb |= 1u << k;
+        }
+        o.hv = o.v ? (uint32_t)(31 - __builtin_clz(o.v)) : 0u;
+        o.z = g.z;
+        o.c = g.m[0].x;
+        o.sn = g.m[0].y;
+        o.ph = g.m[1];
+        out.push_back(o);
+    }
+}
+
+static int launch_span(qfe_state_s* st, const SpanDesc& hd, const SpanDesc* dd, const uint64_t* ctab, const SpanOp* sops) {
+    Context& cx = ctx();
+    const uint32_t bits = hd.L + hd.D;
+    if (bits > (uint32_t)kSpanK || bits > (uint32_t)st->n) return fail(QFE_ERR_STATE, "internal: span tile out of range");
+    const size_t lds = sizeof(double2) << bits;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_span), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(double2) << kSpanK));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_span, dim3((unsigned)(st->N >> bits)), dim3(kSpanTPB), lds, cx.stream, st->d, dd, ctab, sops);
+    QFE_HIP(hipGetLastError());
+    cx.counters.fused++;
+    cx.counters.bytes += 32.0 * (double)st->N;
+    return QFE_OK;
 }
 
 template <int KQ>
@@ -871,6 +1012,10 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     std::vector<double2> mats;
     std::vector<OrbitDesc> odescs(passes.size());
     std::vector<OrbitOp> oops;
+    std::vector<SpanDesc> sdescs;
+    std::vector<uint64_t> ctabs;
+    std::vector<SpanOp> sops;
+    std::vector<size_t> span_d(passes.size(), 0), span_c(passes.size(), 0);
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const Pass& p = passes[pi];
         if (p.kind == P_SINGLE) {
@@ -884,6 +1029,14 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (p.kind == P_ORBIT) {
             pass_op0[pi] = oops.size();
             build_orbit(p, ops, odescs[pi], oops);
+            continue;
+        }
+        if (p.kind == P_SPAN) {
+            span_d[pi] = sdescs.size();
+            span_c[pi] = ctabs.size();
+            pass_op0[pi] = sops.size();
+            sdescs.emplace_back();
+            build_span(p, ops, sdescs.back(), ctabs, sops);
             continue;
         }
         // extend the tile with the lowest free qubits up to K bits
@@ -924,9 +1077,15 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     const size_t bytes_mat = mats.size() * sizeof(double2);
     const size_t off_orb = (off_mat + bytes_mat + 255) & ~(size_t)255;
     const size_t bytes_orb = oops.size() * sizeof(OrbitOp);
-    const size_t total = off_orb + bytes_orb;
+    const size_t off_sd = (off_orb + bytes_orb + 255) & ~(size_t)255;
+    const size_t bytes_sd = sdescs.size() * sizeof(SpanDesc);
+    const size_t off_ct = (off_sd + bytes_sd + 255) & ~(size_t)255;
+    const size_t bytes_ct = ctabs.size() * sizeof(uint64_t);
+    const size_t off_so = (off_ct + bytes_ct + 255) & ~(size_t)255;
+    const size_t bytes_so = sops.size() * sizeof(SpanOp);
+    const size_t total = off_so + bytes_so;
     char* dev = nullptr;
-    if (bytes_ops || bytes_mat || bytes_orb) {
+    if (bytes_ops || bytes_mat || bytes_orb || bytes_sd) {
         void* scr = nullptr;
         int rc = scratch_reserve(total, &scr);
         if (rc) return rc;
@@ -938,6 +1097,9 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (bytes_ops) std::memcpy(host + off_ops, dops.data(), bytes_ops);
         if (bytes_mat) std::memcpy(host + off_mat, mats.data(), bytes_mat);
         if (bytes_orb) std::memcpy(host + off_orb, oops.data(), bytes_orb);
+        if (bytes_sd) std::memcpy(host + off_sd, sdescs.data(), bytes_sd);
+        if (bytes_ct) std::memcpy(host + off_ct, ctabs.data(), bytes_ct);
+        if (bytes_so) std::memcpy(host + off_so, sops.data(), bytes_so);
         rc = staging_commit(dev, total);
         if (rc) return rc;
     }
@@ -950,6 +1112,10 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         } else if (p.kind == P_ORBIT) {
             const OrbitOp* oo = reinterpret_cast<const OrbitOp*>(dev + off_orb) + pass_op0[pi];
             rc = launch_orbit(st, (int)p.span.b.size(), odescs[pi], oo, (uint32_t)p.ops.size());
+        } else if (p.kind == P_SPAN) {
+            rc = launch_span(st, sdescs[span_d[pi]], reinterpret_cast<const SpanDesc*>(dev + off_sd) + span_d[pi],
+                             reinterpret_cast<const uint64_t*>(dev + off_ct) + span_c[pi],
+                             reinterpret_cast<const SpanOp*>(dev + off_so) + pass_op0[pi]);
         } else {
             const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops) + pass_op0[pi];
             rc = launch_fused(st, pdescs[pass_pd[pi]], dop);

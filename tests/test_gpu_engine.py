@@ -324,3 +324,44 @@ def test_large_state_norm_and_replica(qf, oracle, n):
         for theta, term in strings:
             psi = oracle.exp_pauli(psi, -1j * theta, term)
         assert np.max(np.abs(amps - psi)) < AMP_TOL
+
+
+@pytest.mark.parametrize("tuning", [(11, 4, 24), (11, 2, 64), (11, 6, 9)])
+def test_span_tile_runs(qf, oracle, tuning):
+    # 18 qubits: runs of exponentials whose flip masks span up to 8
+    # dimensions go to LDS span tiles; mixed with repeated masks, masks that
+    # combine earlier ones, pure Z strings and masks confined to the low bits
+    n = 18
+    qf.set_tuning(*tuning)
+    r = np.random.default_rng(1234)
+    seq = []
+    for _ in range(90):
+        kind = int(r.integers(8))
+        if kind == 0 and seq:
+            seq.append((float(r.normal()), seq[int(r.integers(len(seq)))][1]))
+        elif kind == 1:
+            qs = sorted(int(v) for v in r.choice(n, size=3, replace=False))
+            seq.append((float(r.normal()), [("Z", q) for q in qs]))
+        elif kind == 2:
+            qs = sorted(int(v) for v in r.choice(4, size=2, replace=False))
+            seq.append((float(r.normal()), [(("X", "Y")[int(r.integers(2))], q) for q in qs]))
+        else:
+            w = int(r.integers(2, 9))
+            qs = sorted(int(v) for v in r.choice(n, size=w, replace=False))
+            seq.append((float(r.normal()), [(("X", "Y", "Z")[int(r.integers(3))], q) for q in qs]))
+    op = W.exp_operator(qf, seq)
+    U, phase = qf.trotterize(op)
+    assert phase == 1.0
+    psi = rand_state(n, 21)
+    qc = qf.Computer(n)
+    qc.set_coeff_vec(psi)
+    qf.reset_counters()
+    qc.apply_circuit(U)
+    passes = qf.get_counters()["fused_passes"]
+    expect = psi.copy()
+    for theta, term in seq:
+        expect = oracle.exp_pauli(expect, -1j * theta, term)
+    assert np.max(np.abs(qc.get_coeff_vec() - expect)) < AMP_TOL
+    assert passes < len(seq) / 3
+    qc.apply_circuit(U.adjoint())
+    assert np.max(np.abs(qc.get_coeff_vec() - psi)) < AMP_TOL
