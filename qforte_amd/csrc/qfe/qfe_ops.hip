@@ -331,13 +331,29 @@ struct PassDesc {
     uint32_t nlow;           // trailing contiguous tile bits 0..nlow-1
     uint32_t nops;
     uint32_t nrest;          // number of non-tile qubits
+    uint32_t phase0, nphases;        // this pass's phases in the phase table
     uint8_t hipos[kMaxTileQubits];   // global positions of tile bits nlow..k-1
     uint8_t restpos[64];             // global positions of non-tile bits, ascending
 };
 
+// A pass is a sequence of phases.  A register phase holds consecutive U1,
+// DIAG1 and PEXP ops whose mixing bits fit three tile bits r0 < r1 < r2:
+// every thread loads the 8 amplitudes of a work item (all tile bits fixed but
+// r0..r2) from LDS once, applies all of the phase's ops in registers and
+// writes them back, so the tile is read and written once per phase instead
+// of once per op.  Any other op is a phase of its own, applied pair by pair
+// in LDS.  For register-phase ops DevOp::pad holds the flip in work-item
+// coordinates (bit k <-> tile bit r_k).
+struct Phase {
+    uint32_t op0, nops;
+    uint32_t reg;
+    uint32_t r[3];
+};
+
 template <int KQ>
 __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
-                                                     const DevOp* __restrict__ ops, uint32_t nops) {
+                                                     const DevOp* __restrict__ ops, const Phase* __restrict__ phases,
+                                                     uint32_t nphases) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
     constexpr uint32_t T = 1u << KQ;
@@ -363,9 +379,66 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
     const uint32_t lowmask = (1u << nlow) - 1u;
 #pragma unroll
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) tile[l] = s[base | hioff[l >> nlow] | (l & lowmask)];
-    for (uint32_t oi = 0; oi < nops; ++oi) {
+    for (uint32_t phi = 0; phi < nphases; ++phi) {
         __syncthreads();
-        const DevOp& o = ops[oi];
+        const Phase P = phases[phi];
+        if (P.reg) {
+            const uint32_t b0 = 1u << P.r[0], b1 = 1u << P.r[1], b2 = 1u << P.r[2];
+            for (uint32_t it = threadIdx.x; it < (T >> 3); it += kFusedTPB) {
+                const uint32_t eb = (uint32_t)ins0(ins0(ins0(it, P.r[0]), P.r[1]), P.r[2]);
+                uint32_t e[8];
+                double2 a[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    e[i] = eb | ((i & 1) ? b0 : 0u) | ((i & 2) ? b1 : 0u) | ((i & 4) ? b2 : 0u);
+                    a[i] = tile[e[i]];
+                }
+                for (uint32_t oi = P.op0; oi < P.op0 + P.nops; ++oi) {
+                    const DevOp& o = ops[oi];
+                    if ((base & o.omask) != o.oval) continue;  // uniform across the block
+                    const uint32_t cm = (uint32_t)o.cmask, cv = (uint32_t)o.cval;
+                    if (o.kind == K_U1) {
+                        double2 p[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) p[i] = a[i];
+                        xor_permute<3>(p, o.pad);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            if ((e[i] & cm) != cv) continue;
+                            a[i] = ((e[i] >> o.t) & 1u) ? cmix(o.m[2], p[i], o.m[3], a[i]) : cmix(o.m[0], a[i], o.m[1], p[i]);
+                        }
+                    } else if (o.kind == K_DIAG1) {
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            if ((e[i] & cm) != cv) continue;
+                            const bool hi = o.tin ? ((e[i] >> o.t) & 1u) : ((base >> o.t) & 1ull);
+                            a[i] = cmul(hi ? o.m[1] : o.m[0], a[i]);
+                        }
+                    } else {  // K_PEXP with its flip inside r0..r2
+                        const double c = o.m[0].x, sn = o.m[0].y;
+                        const double2 iph = make_double2(-o.m[1].y, o.m[1].x);
+                        const bool bneg = __popcll(base & o.ozmask) & 1;
+                        const uint32_t zl = (uint32_t)o.z, xl = (uint32_t)o.x;
+                        double2 p[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) p[i] = a[i];
+                        xor_permute<3>(p, o.pad);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            // a_i' = c a_i + i sin ph (-1)^{(e_i ^ x).z} a_{i ^ v}
+                            const bool neg = ((__popc((e[i] ^ xl) & zl) & 1) != 0) != bneg;
+                            const double2 t = cmul(iph, p[i]);
+                            const double ss = neg ? -sn : sn;
+                            a[i] = make_double2(c * a[i].x + ss * t.x, c * a[i].y + ss * t.y);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) tile[e[i]] = a[i];
+            }
+            continue;
+        }
+        const DevOp& o = ops[P.op0];
         if ((base & o.omask) != o.oval) continue;  // uniform across the block
         switch (o.kind) {
             case K_U1: {
@@ -909,7 +982,8 @@ static int launch_span(qfe_state_s* st, const SpanDesc& hd, const SpanDesc* dd, 
 }
 
 template <int KQ>
-static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, hipStream_t st) {
+static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, const Phase* ph,
+                           hipStream_t st) {
     const size_t lds = (size_t)sizeof(double2) << KQ;
     static bool attr_set = false;  // large tiles need the dynamic-LDS limit raised once
     if (!attr_set) {
@@ -917,27 +991,27 @@ static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, con
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops,
-                       pd.nops);
+    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops, ph,
+                       pd.nphases);
 }
 
-static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) {
+static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops, const Phase* ph) {
     Context& cx = ctx();
     const int k = (int)pd.k;
     const uint64_t nblocks = st->N >> k;
     switch (k) {
-        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, ph, cx.stream); break;
         default: return fail(QFE_ERR_ARG, "internal: fused tile size out of range");
     }
     QFE_HIP(hipGetLastError());
@@ -995,6 +1069,70 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
     ops.swap(out);
 }
 
+// Split the localized ops [first, first + count) of a tile pass into phases:
+// runs of U1 / DIAG1 / PEXP whose mixing bits fit three tile bits become
+// register phases; any other op is a phase of its own.  Phase op indices are
+// absolute in dops.
+static void build_phases(std::vector<DevOp>& dops, size_t first, uint32_t count, int k, std::vector<Phase>& out) {
+    Phase cur{};
+    uint32_t rmask = 0;
+    bool open = false;
+    auto flush = [&]() {
+        if (!open) return;
+        // pad to three register bits with the lowest free tile bits
+        for (int q = 0; q < k && __builtin_popcount(rmask) < 3; ++q) rmask |= 1u << q;
+        int r = 0;
+        for (int q = 0; q < k && r < 3; ++q)
+            if ((rmask >> q) & 1u) cur.r[r++] = (uint32_t)q;
+        for (uint32_t i = cur.op0; i < cur.op0 + cur.nops; ++i) {
+            DevOp& o = dops[i];
+            const uint32_t m = o.kind == K_U1 ? (1u << o.t) : (o.kind == K_PEXP ? (uint32_t)o.x : 0u);
+            uint32_t v = 0;
+            for (int a = 0; a < 3; ++a)
+                if ((m >> cur.r[a]) & 1u) v |= 1u << a;
+            o.pad = v;
+        }
+        out.push_back(cur);
+        open = false;
+        rmask = 0;
+    };
+    for (uint32_t j = 0; j < count; ++j) {
+        const size_t i = first + j;
+        const DevOp& o = dops[i];
+        uint32_t m = 0;
+        bool eligible = k >= 3;
+        if (o.kind == K_U1)
+            m = 1u << o.t;
+        else if (o.kind == K_DIAG1)
+            m = 0;
+        else if (o.kind == K_PEXP)
+            m = (uint32_t)o.x;
+        else
+            eligible = false;
+        if (eligible && __builtin_popcount(m) > 3) eligible = false;
+        if (!eligible) {
+            flush();
+            Phase single{};
+            single.op0 = (uint32_t)i;
+            single.nops = 1;
+            single.reg = 0;
+            out.push_back(single);
+            continue;
+        }
+        if (open && __builtin_popcount(rmask | m) > 3) flush();
+        if (!open) {
+            cur = Phase{};
+            cur.op0 = (uint32_t)i;
+            cur.nops = 0;
+            cur.reg = 1;
+            open = true;
+        }
+        rmask |= m;
+        cur.nops++;
+    }
+    flush();
+}
+
 int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     if (ops.empty()) return QFE_OK;
     Context& cx = ctx();
@@ -1012,6 +1150,7 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     std::vector<double2> mats;
     std::vector<OrbitDesc> odescs(passes.size());
     std::vector<OrbitOp> oops;
+    std::vector<Phase> phases;
     std::vector<SpanDesc> sdescs;
     std::vector<uint64_t> ctabs;
     std::vector<SpanOp> sops;
@@ -1065,10 +1204,13 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         }
         pd.nrest = (uint32_t)ri;
         pd.nops = (uint32_t)p.ops.size();
-        pass_pd[pi] = pdescs.size();
-        pdescs.push_back(pd);
         pass_op0[pi] = dops.size();
         for (int oi : p.ops) dops.push_back(localize(ops[oi].op, b, local_of));
+        pd.phase0 = (uint32_t)phases.size();
+        build_phases(dops, pass_op0[pi], pd.nops, (int)pd.k, phases);
+        pd.nphases = (uint32_t)phases.size() - pd.phase0;
+        pass_pd[pi] = pdescs.size();
+        pdescs.push_back(pd);
     }
     const size_t bytes_pd = pdescs.size() * sizeof(PassDesc);
     const size_t off_ops = (bytes_pd + 255) & ~(size_t)255;
@@ -1083,7 +1225,9 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     const size_t bytes_ct = ctabs.size() * sizeof(uint64_t);
     const size_t off_so = (off_ct + bytes_ct + 255) & ~(size_t)255;
     const size_t bytes_so = sops.size() * sizeof(SpanOp);
-    const size_t total = off_so + bytes_so;
+    const size_t off_ph = (off_so + bytes_so + 255) & ~(size_t)255;
+    const size_t bytes_ph = phases.size() * sizeof(Phase);
+    const size_t total = off_ph + bytes_ph;
     char* dev = nullptr;
     if (bytes_ops || bytes_mat || bytes_orb || bytes_sd) {
         void* scr = nullptr;
@@ -1100,6 +1244,7 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (bytes_sd) std::memcpy(host + off_sd, sdescs.data(), bytes_sd);
         if (bytes_ct) std::memcpy(host + off_ct, ctabs.data(), bytes_ct);
         if (bytes_so) std::memcpy(host + off_so, sops.data(), bytes_so);
+        if (bytes_ph) std::memcpy(host + off_ph, phases.data(), bytes_ph);
         rc = staging_commit(dev, total);
         if (rc) return rc;
     }
@@ -1117,8 +1262,9 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
                              reinterpret_cast<const uint64_t*>(dev + off_ct) + span_c[pi],
                              reinterpret_cast<const SpanOp*>(dev + off_so) + pass_op0[pi]);
         } else {
-            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops) + pass_op0[pi];
-            rc = launch_fused(st, pdescs[pass_pd[pi]], dop);
+            const PassDesc& pd = pdescs[pass_pd[pi]];
+            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops);  // phases index ops of the whole call
+            rc = launch_fused(st, pd, dop, reinterpret_cast<const Phase*>(dev + off_ph) + pd.phase0);
         }
         if (rc) return rc;
     }
