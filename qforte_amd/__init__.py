@@ -35,6 +35,7 @@ from ._core import (  # noqa: F401
     find_irrep,
     gate,
     get_counters,
+    get_pass_tuning,
     get_tuning,
     inner_product,
     norm2,
@@ -43,6 +44,7 @@ from ._core import (  # noqa: F401
     prepare_computer_from_circuit,
     prepare_computer_from_gates,
     reset_counters,
+    set_pass_tuning,
     set_tuning,
     stream_handle,
     synchronize,

@@ -668,6 +668,95 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
 }
 
 // ---------------------------------------------------------------------------
+// Device: register orbits of 4 + Q dimensions spread over 2^Q lanes
+//
+// An orbit of 2^(4+Q) amplitudes is held by 2^Q lanes of one wave, 16
+// amplitudes each: lane group h owns the elements i = h * 16 + i'.  An
+// exponential whose orbit flip v keeps the top Q bits (v >> 4 == 0) pairs
+// elements inside a lane as in k_orbit; otherwise the partner of i' lives in
+// the lane whose group is h ^ (v >> 4), at i' ^ (v & 15): the lane permutes
+// its own 16 values by v & 15 (xor butterfly) and swaps them with that lane
+// by a cross-lane shuffle.  Lanes of one group hold consecutive orbits, so a
+// wave reads runs of 2^(6-Q) consecutive amplitudes per element.
+// ---------------------------------------------------------------------------
+
+struct OrbitOp2 {
+    uint32_t v;
+    uint32_t pad;
+    uint64_t smask;  // bit i = parity(c_i & z), i over 2^(4+Q) elements
+    uint64_t z;
+    double c, sn;
+    double2 ph;
+};
+
+__device__ __forceinline__ double2 shfl_xor_d2(double2 v, int mask) {
+    return make_double2(__shfl_xor(v.x, mask, kWave), __shfl_xor(v.y, mask, kWave));
+}
+
+template <int Q>
+__global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__ s, uint64_t norbit,
+                                                           const uint64_t* __restrict__ basis,  // 4 + Q, pivots ascending
+                                                           const uint32_t* __restrict__ piv,
+                                                           const OrbitOp2* __restrict__ ops, uint32_t nops) {
+    constexpr int M = 4 + Q;
+    constexpr uint32_t OPW = kWave >> Q;  // orbits per wave
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t wave = ((uint64_t)blockIdx.x * kOrbitTPB + threadIdx.x) / kWave;
+    const uint64_t o = wave * OPW + (lane & (OPW - 1));
+    const uint32_t h = lane / OPW;
+    // all lanes of a wave stay to the end (the shuffles need their partners);
+    // lanes past the last orbit neither load nor store
+    const bool live = o < norbit;
+    uint64_t j = live ? o : 0;
+    for (int k = 0; k < M; ++k) j = ins0(j, piv[k]);
+    uint64_t hc = 0;
+    for (int k = 0; k < Q; ++k)
+        if ((h >> k) & 1u) hc ^= basis[4 + k];
+    const uint64_t b0 = basis[0], b1 = basis[1], b2 = basis[2], b3 = basis[3];
+    uint64_t off[16];
+    double2 a[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        off[i] = j ^ hc ^ ((i & 1) ? b0 : 0ull) ^ ((i & 2) ? b1 : 0ull) ^ ((i & 4) ? b2 : 0ull) ^ ((i & 8) ? b3 : 0ull);
+        a[i] = live ? s[off[i]] : make_double2(0.0, 0.0);
+    }
+    const uint32_t hbase = h * 16u;
+    for (uint32_t oi = 0; oi < nops; ++oi) {
+        const OrbitOp2 op = ops[oi];
+        const uint64_t sm = op.smask ^ ((__popcll(j & op.z) & 1) ? ~0ull : 0ull);
+        const double2 iph = make_double2(-op.ph.y, op.ph.x);  // i * phase
+        if (op.v == 0) {
+            const double2 fp = make_double2(op.c - op.sn * op.ph.y, op.sn * op.ph.x);
+            const double2 fm = make_double2(op.c + op.sn * op.ph.y, -op.sn * op.ph.x);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) a[i] = cmul(((sm >> (hbase + i)) & 1ull) ? fm : fp, a[i]);
+            continue;
+        }
+        double2 p[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) p[i] = a[i];
+        xor_permute<4>(p, op.v & 15u);
+        const uint32_t vq = op.v >> 4;
+        if (vq) {
+            const int lm = (int)(vq * OPW);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) p[i] = shfl_xor_d2(p[i], lm);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            // a_i' = cos a_i + i sin ph (-1)^{s(i ^ v)} a_{i ^ v}, i = h * 16 + i'
+            const double sn = ((sm >> ((hbase + i) ^ op.v)) & 1ull) ? -op.sn : op.sn;
+            const double2 t = cmul(iph, p[i]);
+            a[i] = make_double2(op.c * a[i].x + sn * t.x, op.c * a[i].y + sn * t.y);
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s[off[i]] = a[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Device: span tile for longer runs of Pauli exponentials
 //
 // The flip masks of a run span D dimensions with a reduced basis b_0..b_{D-1}
@@ -845,11 +934,11 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
         if (first >= ops.size()) break;
         Pass tile, orbit, span;
         grow_tile(ops, done, first, K, lowmask, all, tu.max_ops, tile);
-        grow_orbit(ops, done, first, all, tu.max_ops, orbit);
+        grow_orbit(ops, done, first, all, tu.max_ops, orbit, P_ORBIT, std::max(1, std::min(tu.orbit_dims, kOrbitMax + 2)));
         // span tiles: low L bits contiguous, up to kSpanK - L dimensions, and
         // at least 2^8 tiles so the grid fills the chip
-        const int SL = std::min(tu.low_qubits, 6);
-        const int SD = std::min(kSpanK - SL, n - SL - 8);
+        const int SL = tu.span_low;
+        const int SD = std::min(std::min(kSpanK - SL, n - SL - 8), tu.span_dims);
         if (SD > kOrbitMax) grow_orbit(ops, done, first, all, tu.max_ops, span, P_SPAN, SD, SL);
         span.span_L = SL;
         Pass p;
@@ -928,6 +1017,63 @@ static void build_orbit(const Pass& p, const std::vector<HostOp>& ops, OrbitDesc
         o.ph = g.m[1];
         out.push_back(o);
     }
+}
+
+// Lane-orbit descriptor (basis sorted by pivot) and per-op data for an orbit
+// pass of 5 or 6 dimensions.
+struct LaneDesc {
+    uint64_t b[kOrbitMax + 2];
+    uint32_t piv[kOrbitMax + 2];
+};
+
+static void build_orbit_lanes(const Pass& p, const std::vector<HostOp>& ops, LaneDesc& d, std::vector<OrbitOp2>& out) {
+    std::vector<uint64_t> b = p.span.b;
+    std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
+    const int M = (int)b.size();
+    d = LaneDesc{};
+    for (int k = 0; k < M; ++k) {
+        d.b[k] = b[k];
+        d.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+    }
+    std::vector<uint64_t> c(1u << M);
+    for (int i = 0; i < (1 << M); ++i) {
+        uint64_t x = 0;
+        for (int k = 0; k < M; ++k)
+            if ((i >> k) & 1) x ^= b[k];
+        c[i] = x;
+    }
+    for (int oi : p.ops) {
+        const DevOp& g = ops[oi].op;
+        OrbitOp2 o{};
+        for (int k = 0; k < M; ++k)
+            if ((g.x >> d.piv[k]) & 1ull) o.v |= 1u << k;
+        for (int i = 0; i < (1 << M); ++i)
+            if (__builtin_popcountll(c[i] & g.z) & 1) o.smask |= 1ull << i;
+        o.z = g.z;
+        o.c = g.m[0].x;
+        o.sn = g.m[0].y;
+        o.ph = g.m[1];
+        out.push_back(o);
+    }
+}
+
+static int launch_orbit_lanes(qfe_state_s* st, int M, const LaneDesc* dd, const OrbitOp2* ops, uint32_t nops) {
+    Context& cx = ctx();
+    const uint64_t norbit = st->N >> M;
+    const uint64_t threads = norbit << (M - kOrbitMax);
+    const unsigned blocks = (unsigned)((threads + kOrbitTPB - 1) / kOrbitTPB);
+    const uint64_t* b = reinterpret_cast<const uint64_t*>(dd);
+    const uint32_t* piv = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(dd) + offsetof(LaneDesc, piv));
+    if (M == 5)
+        hipLaunchKernelGGL(k_orbit_lanes<1>, dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+    else if (M == 6)
+        hipLaunchKernelGGL(k_orbit_lanes<2>, dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+    else
+        return fail(QFE_ERR_STATE, "internal: lane orbit dimension out of range");
+    QFE_HIP(hipGetLastError());
+    cx.counters.fused++;
+    cx.counters.bytes += 32.0 * (double)st->N;
+    return QFE_OK;
 }
 
 // Span descriptor, orbit offsets and per-op data for a finished span pass.
@@ -1073,7 +1219,8 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
 // runs of U1 / DIAG1 / PEXP whose mixing bits fit three tile bits become
 // register phases; any other op is a phase of its own.  Phase op indices are
 // absolute in dops.
-static void build_phases(std::vector<DevOp>& dops, size_t first, uint32_t count, int k, std::vector<Phase>& out) {
+static void build_phases(std::vector<DevOp>& dops, size_t first, uint32_t count, int k, bool reg,
+                         std::vector<Phase>& out) {
     Phase cur{};
     uint32_t rmask = 0;
     bool open = false;
@@ -1100,7 +1247,7 @@ static void build_phases(std::vector<DevOp>& dops, size_t first, uint32_t count,
         const size_t i = first + j;
         const DevOp& o = dops[i];
         uint32_t m = 0;
-        bool eligible = k >= 3;
+        bool eligible = reg && k >= 3;
         if (o.kind == K_U1)
             m = 1u << o.t;
         else if (o.kind == K_DIAG1)
@@ -1151,6 +1298,8 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     std::vector<OrbitDesc> odescs(passes.size());
     std::vector<OrbitOp> oops;
     std::vector<Phase> phases;
+    std::vector<LaneDesc> ldescs;
+    std::vector<OrbitOp2> lops;
     std::vector<SpanDesc> sdescs;
     std::vector<uint64_t> ctabs;
     std::vector<SpanOp> sops;
@@ -1163,6 +1312,13 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
                 single_mat[pi] = mats.size();
                 for (int i = 0; i < 16; ++i) mats.push_back(h.op.m[i]);
             }
+            continue;
+        }
+        if (p.kind == P_ORBIT && p.span.b.size() > (size_t)kOrbitMax) {
+            span_d[pi] = ldescs.size();
+            pass_op0[pi] = lops.size();
+            ldescs.emplace_back();
+            build_orbit_lanes(p, ops, ldescs.back(), lops);
             continue;
         }
         if (p.kind == P_ORBIT) {
@@ -1207,7 +1363,7 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         pass_op0[pi] = dops.size();
         for (int oi : p.ops) dops.push_back(localize(ops[oi].op, b, local_of));
         pd.phase0 = (uint32_t)phases.size();
-        build_phases(dops, pass_op0[pi], pd.nops, (int)pd.k, phases);
+        build_phases(dops, pass_op0[pi], pd.nops, (int)pd.k, cx.tuning.reg_phases != 0, phases);
         pd.nphases = (uint32_t)phases.size() - pd.phase0;
         pass_pd[pi] = pdescs.size();
         pdescs.push_back(pd);
@@ -1227,9 +1383,13 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     const size_t bytes_so = sops.size() * sizeof(SpanOp);
     const size_t off_ph = (off_so + bytes_so + 255) & ~(size_t)255;
     const size_t bytes_ph = phases.size() * sizeof(Phase);
-    const size_t total = off_ph + bytes_ph;
+    const size_t off_ld = (off_ph + bytes_ph + 255) & ~(size_t)255;
+    const size_t bytes_ld = ldescs.size() * sizeof(LaneDesc);
+    const size_t off_lo = (off_ld + bytes_ld + 255) & ~(size_t)255;
+    const size_t bytes_lo = lops.size() * sizeof(OrbitOp2);
+    const size_t total = off_lo + bytes_lo;
     char* dev = nullptr;
-    if (bytes_ops || bytes_mat || bytes_orb || bytes_sd) {
+    if (bytes_ops || bytes_mat || bytes_orb || bytes_sd || bytes_ld) {
         void* scr = nullptr;
         int rc = scratch_reserve(total, &scr);
         if (rc) return rc;
@@ -1245,6 +1405,8 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (bytes_ct) std::memcpy(host + off_ct, ctabs.data(), bytes_ct);
         if (bytes_so) std::memcpy(host + off_so, sops.data(), bytes_so);
         if (bytes_ph) std::memcpy(host + off_ph, phases.data(), bytes_ph);
+        if (bytes_ld) std::memcpy(host + off_ld, ldescs.data(), bytes_ld);
+        if (bytes_lo) std::memcpy(host + off_lo, lops.data(), bytes_lo);
         rc = staging_commit(dev, total);
         if (rc) return rc;
     }
@@ -1254,6 +1416,9 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (p.kind == P_SINGLE) {
             double2* m = single_mat[pi] == (size_t)-1 ? nullptr : reinterpret_cast<double2*>(dev + off_mat) + single_mat[pi];
             rc = launch_single(st, ops[p.ops[0]], m);
+        } else if (p.kind == P_ORBIT && p.span.b.size() > (size_t)kOrbitMax) {
+            rc = launch_orbit_lanes(st, (int)p.span.b.size(), reinterpret_cast<const LaneDesc*>(dev + off_ld) + span_d[pi],
+                                    reinterpret_cast<const OrbitOp2*>(dev + off_lo) + pass_op0[pi], (uint32_t)p.ops.size());
         } else if (p.kind == P_ORBIT) {
             const OrbitOp* oo = reinterpret_cast<const OrbitOp*>(dev + off_orb) + pass_op0[pi];
             rc = launch_orbit(st, (int)p.span.b.size(), odescs[pi], oo, (uint32_t)p.ops.size());
