@@ -430,8 +430,8 @@ def main():
         achieved = 2 * S / (avg_ms * 1e-3) / 1e9
         single_ms = time_single_sweep(qf, W, qc, n)
         roofline = {
-            "kernel": ("k_orbit<4> (register-orbit pass of the exponential stage: %d passes for %d exponentials, "
-                       "2S = %.2f GB per launch)" % (passes, len(strings), 2 * S / 1e9)),
+            "kernel": ("k_orbit_lanes<2> (6-dimension register orbits over 4 lanes; the exponential stage: %d passes "
+                       "for %d exponentials, 2S = %.2f GB per launch)" % (passes, len(strings), 2 * S / 1e9)),
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
