@@ -11,9 +11,10 @@ are resident in HBM when the timed region starts.
 ``value`` is the achieved HBM bandwidth of the whole step: the algorithmic
 bytes of every pass the engine launched (2 * 16 * 2^n per read-write sweep,
 16 * 2^n per read-only expectation pass over one X-mask group) divided by the
-step time.  ``roofline`` reports the dominant kernel (the single Pauli
-exponential sweep) timed with HIP events on the engine stream, against the
-8 TB/s HBM peak.  ``cpu_baseline`` times the reference's own C++ Computer
+step time.  ``roofline`` reports the dominant kernel (the register-orbit
+passes of the exponential stage) timed with HIP events on the engine stream,
+against the 8 TB/s HBM peak.  ``serving`` and ``gradient`` report the UCC
+energy serving leg and the 20-qubit differentiation leg.  ``cpu_baseline`` times the reference's own C++ Computer
 (compiled from its sources into oracle/_ref) single-threaded on a bounded
 26-qubit sample of the same workload.
 
@@ -31,11 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per k_orbit<4> launch at 30 qubits from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch: 8388654 and 16777235), each scaled by the same
-# counter's reading on k_scale, a plain sweep that reads and writes exactly
-# 16 GiB (factors 2.0000 and 1.0000): 33554543 KiB.
-TRAFFIC_PER_LAUNCH = {30: 33554543 * 1024}
+# HBM bytes per k_orbit_lanes<2> launch at 30 qubits from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388672 and 16777229), each
+# scaled by the same counter's reading on k_scale, a plain sweep that reads
+# and writes exactly 16 GiB (factors 2.0000 and 1.0000): 33554573 KiB.
+TRAFFIC_PER_LAUNCH = {30: 33554573 * 1024}
 METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
 
 
