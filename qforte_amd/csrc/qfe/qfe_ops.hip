@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -33,6 +34,15 @@ static inline double2 d2(const qfe_c64& a) { return make_double2(a.re, a.im); }
 // The reference's element threshold (computer.h:172).
 constexpr double kComputeThreshold = 1.0e-16;
 
+// A U1 whose matrix is exactly [[0, 1], [1, 0]] (X, the target half of cX)
+// moves amplitudes without arithmetic; DevOp::hb flags it for the kernels.
+static inline uint32_t is_swap(const double2* m) {
+    return m[0].x == 0.0 && m[0].y == 0.0 && m[3].x == 0.0 && m[3].y == 0.0 && m[1].x == 1.0 && m[1].y == 0.0 &&
+                   m[2].x == 1.0 && m[2].y == 0.0
+               ? 1u
+               : 0u;
+}
+
 static void add_u1(int t, const qfe_c64 u[4], uint64_t cmask, std::vector<HostOp>& out) {
     // u = {m00, m01, m10, m11}; classified like computer.cc:353-422
     const bool diag_nz = cabs(u[0]) + cabs(u[3]) > kComputeThreshold;
@@ -51,6 +61,7 @@ static void add_u1(int t, const qfe_c64 u[4], uint64_t cmask, std::vector<HostOp
     } else {
         h.op.kind = K_U1;
         for (int i = 0; i < 4; ++i) h.op.m[i] = d2(u[i]);
+        h.op.hb = is_swap(h.op.m);
         h.mix = 1ull << t;
     }
     out.push_back(h);
@@ -442,8 +453,18 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
         if ((base & o.omask) != o.oval) continue;  // uniform across the block
         switch (o.kind) {
             case K_U1: {
-                const double2 m[4] = {o.m[0], o.m[1], o.m[2], o.m[3]};
                 const uint32_t tb = 1u << o.t;
+                if (o.hb) {  // X on the target: swap the pair, no arithmetic
+                    for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                        const uint32_t l0 = (uint32_t)ins0(p, o.t);
+                        if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
+                        const double2 a = tile[l0];
+                        tile[l0] = tile[l0 | tb];
+                        tile[l0 | tb] = a;
+                    }
+                    break;
+                }
+                const double2 m[4] = {o.m[0], o.m[1], o.m[2], o.m[3]};
                 for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
                     const uint32_t l0 = (uint32_t)ins0(p, o.t);
                     if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
@@ -1205,6 +1226,7 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
                 } else {
                     p.op.kind = K_U1;
                     for (int i = 0; i < 4; ++i) p.op.m[i] = r[i];
+                    p.op.hb = is_swap(p.op.m);
                     p.mix = 1ull << p.op.t;
                 }
                 continue;
