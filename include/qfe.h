@@ -86,14 +86,10 @@ int qfe_synchronize(void);
  * counterpart (autotuner hook). */
 int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass);
 int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass);
-/* Pass kinds of the planner (no reference counterpart): span tiles for runs of
- * exponentials up to span_max_dims GF(2) dimensions with span_low_bits
- * contiguous low bits (span_max_dims <= 4 disables them), register phases
- * inside LDS tile passes (0/1), and the largest register orbit (1..6
- * dimensions; 5 and 6 spread one orbit over 2 or 4 lanes).  Negative
- * arguments leave a setting unchanged. */
-int qfe_set_pass_tuning(int span_max_dims, int span_low_bits, int register_phases, int orbit_max_dims);
-int qfe_get_pass_tuning(int* span_max_dims, int* span_low_bits, int* register_phases, int* orbit_max_dims);
+/* Largest register orbit of the planner, 1..6 dimensions (5 and 6 spread one
+ * orbit over 2 or 4 lanes of a wave); no reference counterpart (autotuner hook). */
+int qfe_set_pass_tuning(int orbit_max_dims);
+int qfe_get_pass_tuning(int* orbit_max_dims);
 /* HIP events on the library stream, for timing kernels from the host
  * (no reference counterpart). */
 int qfe_event_create(void** event);

@@ -634,13 +634,11 @@ PYBIND11_MODULE(_core, m) {
     });
     m.def("set_tuning", [](int k, int l, int ops) { check(qfe_set_tuning(k, l, ops), "set_tuning"); }, "tile_qubits"_a = 0,
           "low_qubits"_a = 0, "max_ops_per_pass"_a = 0);
-    m.def("set_pass_tuning",
-          [](int d, int l, int r, int o) { check(qfe_set_pass_tuning(d, l, r, o), "set_pass_tuning"); },
-          "span_max_dims"_a = -1, "span_low_bits"_a = -1, "register_phases"_a = -1, "orbit_max_dims"_a = -1);
+    m.def("set_pass_tuning", [](int o) { check(qfe_set_pass_tuning(o), "set_pass_tuning"); }, "orbit_max_dims"_a);
     m.def("get_pass_tuning", []() {
-        int d, l, r, o;
-        qfe_get_pass_tuning(&d, &l, &r, &o);
-        return std::make_tuple(d, l, r, o);
+        int o = 0;
+        qfe_get_pass_tuning(&o);
+        return o;
     });
     m.def("get_tuning", []() {
         int k, l, o;

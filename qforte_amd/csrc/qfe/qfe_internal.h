@@ -27,9 +27,6 @@ struct Tuning {
     int tile_qubits = 11;  // k: qubits held by one fused tile (2^k amplitudes in LDS)
     int low_qubits = 4;    // L: lowest qubits always in the tile (contiguous 2^L runs)
     int max_ops = 24;      // ops per fused pass before LDS traffic dominates
-    int span_dims = 0;     // max GF(2) dimensions of a span-tile pass (<= 4: span tiles off)
-    int span_low = 4;      // contiguous low bits of a span tile
-    int reg_phases = 0;    // fused tiles apply runs of small ops in register phases
     int orbit_dims = 6;    // max dimensions of a register orbit (5, 6: spread over 2 or 4 lanes)
 };
 

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "qfe_common.cuh"
@@ -335,6 +336,7 @@ static int launch_single(qfe_state_s* st, const HostOp& h, double2* dev_mat) {
 
 constexpr int kFusedTPB = 256;
 constexpr int kMaxTileQubits = 13;  // 2^13 * 16 B = 128 KiB of LDS
+constexpr uint64_t kFusedGrid = 1024;  // persistent workgroups of a fused pass (about 4 per CU)
 
 struct PassDesc {
     uint64_t bmask;          // tile qubits (global mask)
@@ -342,42 +344,49 @@ struct PassDesc {
     uint32_t nlow;           // trailing contiguous tile bits 0..nlow-1
     uint32_t nops;
     uint32_t nrest;          // number of non-tile qubits
-    uint32_t phase0, nphases;        // this pass's phases in the phase table
     uint8_t hipos[kMaxTileQubits];   // global positions of tile bits nlow..k-1
     uint8_t restpos[64];             // global positions of non-tile bits, ascending
 };
 
-// A pass is a sequence of phases.  A register phase holds consecutive U1,
-// DIAG1 and PEXP ops whose mixing bits fit three tile bits r0 < r1 < r2:
-// every thread loads the 8 amplitudes of a work item (all tile bits fixed but
-// r0..r2) from LDS once, applies all of the phase's ops in registers and
-// writes them back, so the tile is read and written once per phase instead
-// of once per op.  Any other op is a phase of its own, applied pair by pair
-// in LDS.  For register-phase ops DevOp::pad holds the flip in work-item
-// coordinates (bit k <-> tile bit r_k).
-struct Phase {
-    uint32_t op0, nops;
-    uint32_t reg;
-    uint32_t r[3];
-};
+// Per-thread prefetch of a tile into registers with compile-time indices only
+// (fold expressions), so the array stays in VGPRs across the tile loop.
+template <uint32_t T, uint32_t... R>
+__device__ __forceinline__ void tile_fetch(double2 (&nxt)[sizeof...(R)], const double2* __restrict__ s, uint64_t base,
+                                           const uint64_t* hioff, uint32_t nlow, uint32_t lowmask,
+                                           std::integer_sequence<uint32_t, R...>) {
+    const uint32_t t = threadIdx.x;
+    ((nxt[R] = (t + R * kFusedTPB < T) ? s[base | hioff[(t + R * kFusedTPB) >> nlow] | ((t + R * kFusedTPB) & lowmask)]
+                                       : make_double2(0.0, 0.0)),
+     ...);
+}
+
+template <uint32_t T, uint32_t... R>
+__device__ __forceinline__ void tile_put(const double2 (&nxt)[sizeof...(R)], double2* tile,
+                                         std::integer_sequence<uint32_t, R...>) {
+    const uint32_t t = threadIdx.x;
+    ((t + R * kFusedTPB < T ? (void)(tile[t + R * kFusedTPB] = nxt[R]) : (void)0), ...);
+}
 
 template <int KQ>
 __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
-                                                     const DevOp* __restrict__ ops, const Phase* __restrict__ phases,
-                                                     uint32_t nphases) {
+                                                     const DevOp* __restrict__ ops, uint32_t nops, uint64_t ntiles) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
     constexpr uint32_t T = 1u << KQ;
+    constexpr uint32_t PER = T >= kFusedTPB ? T / kFusedTPB : 1u;  // amplitudes per thread
     const uint32_t nhi = KQ - nlow;
     const uint64_t himask = bmask & ~((1ull << nlow) - 1ull);
-    // tile base: blockIdx with a zero bit inserted at every tile qubit
+    // tile base: the tile number with a zero bit inserted at every tile qubit
     // (the low nlow tile qubits are a plain shift); pure scalar ALU
-    uint64_t base = (uint64_t)blockIdx.x << nlow;
-    for (uint64_t m = himask; m; m &= m - 1) {
-        const uint32_t p = (uint32_t)__builtin_ctzll(m);
-        const uint64_t lo = base & ((1ull << p) - 1ull);
-        base = ((base ^ lo) << 1) | lo;
-    }
+    auto tile_base = [&](uint64_t tn) {
+        uint64_t b = tn << nlow;
+        for (uint64_t m = himask; m; m &= m - 1) {
+            const uint32_t p = (uint32_t)__builtin_ctzll(m);
+            const uint64_t lo = b & ((1ull << p) - 1ull);
+            b = ((b ^ lo) << 1) | lo;
+        }
+        return b;
+    };
     // offsets of the high tile qubits (2^nhi entries, nhi <= 9)
     for (uint32_t h = threadIdx.x; h < (1u << nhi); h += kFusedTPB) {
         uint64_t off = 0;
@@ -388,68 +397,21 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
     }
     __syncthreads();
     const uint32_t lowmask = (1u << nlow) - 1u;
-#pragma unroll
-    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) tile[l] = s[base | hioff[l >> nlow] | (l & lowmask)];
-    for (uint32_t phi = 0; phi < nphases; ++phi) {
+    // Persistent blocks: tiles blockIdx.x, + gridDim.x, ...  The next tile is
+    // loaded into registers while the current one is worked on in LDS.
+    uint64_t tcur = blockIdx.x;
+    uint64_t base = tcur < ntiles ? tile_base(tcur) : 0;
+    double2 nxt[PER];
+    using Seq = std::make_integer_sequence<uint32_t, PER>;
+    if (tcur < ntiles) tile_fetch<T>(nxt, s, base, hioff, nlow, lowmask, Seq{});
+    while (tcur < ntiles) {
+        tile_put<T>(nxt, tile, Seq{});
+        const uint64_t tnext = tcur + gridDim.x;
+        const uint64_t bnext = tnext < ntiles ? tile_base(tnext) : 0;
+        if (tnext < ntiles) tile_fetch<T>(nxt, s, bnext, hioff, nlow, lowmask, Seq{});
+    for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
-        const Phase P = phases[phi];
-        if (P.reg) {
-            const uint32_t b0 = 1u << P.r[0], b1 = 1u << P.r[1], b2 = 1u << P.r[2];
-            for (uint32_t it = threadIdx.x; it < (T >> 3); it += kFusedTPB) {
-                const uint32_t eb = (uint32_t)ins0(ins0(ins0(it, P.r[0]), P.r[1]), P.r[2]);
-                uint32_t e[8];
-                double2 a[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    e[i] = eb | ((i & 1) ? b0 : 0u) | ((i & 2) ? b1 : 0u) | ((i & 4) ? b2 : 0u);
-                    a[i] = tile[e[i]];
-                }
-                for (uint32_t oi = P.op0; oi < P.op0 + P.nops; ++oi) {
-                    const DevOp& o = ops[oi];
-                    if ((base & o.omask) != o.oval) continue;  // uniform across the block
-                    const uint32_t cm = (uint32_t)o.cmask, cv = (uint32_t)o.cval;
-                    if (o.kind == K_U1) {
-                        double2 p[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) p[i] = a[i];
-                        xor_permute<3>(p, o.pad);
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            if ((e[i] & cm) != cv) continue;
-                            a[i] = ((e[i] >> o.t) & 1u) ? cmix(o.m[2], p[i], o.m[3], a[i]) : cmix(o.m[0], a[i], o.m[1], p[i]);
-                        }
-                    } else if (o.kind == K_DIAG1) {
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            if ((e[i] & cm) != cv) continue;
-                            const bool hi = o.tin ? ((e[i] >> o.t) & 1u) : ((base >> o.t) & 1ull);
-                            a[i] = cmul(hi ? o.m[1] : o.m[0], a[i]);
-                        }
-                    } else {  // K_PEXP with its flip inside r0..r2
-                        const double c = o.m[0].x, sn = o.m[0].y;
-                        const double2 iph = make_double2(-o.m[1].y, o.m[1].x);
-                        const bool bneg = __popcll(base & o.ozmask) & 1;
-                        const uint32_t zl = (uint32_t)o.z, xl = (uint32_t)o.x;
-                        double2 p[8];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) p[i] = a[i];
-                        xor_permute<3>(p, o.pad);
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            // a_i' = c a_i + i sin ph (-1)^{(e_i ^ x).z} a_{i ^ v}
-                            const bool neg = ((__popc((e[i] ^ xl) & zl) & 1) != 0) != bneg;
-                            const double2 t = cmul(iph, p[i]);
-                            const double ss = neg ? -sn : sn;
-                            a[i] = make_double2(c * a[i].x + ss * t.x, c * a[i].y + ss * t.y);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i) tile[e[i]] = a[i];
-            }
-            continue;
-        }
-        const DevOp& o = ops[P.op0];
+        const DevOp& o = ops[oi];
         if ((base & o.omask) != o.oval) continue;  // uniform across the block
         switch (o.kind) {
             case K_U1: {
@@ -545,6 +507,10 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
     }
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) s[base | hioff[l >> nlow] | (l & lowmask)] = tile[l];
+    __syncthreads();  // the tile's LDS is rewritten by the next iteration
+    tcur = tnext;
+    base = bnext;
+    }
 }
 
 // Map a global mask to tile-local bit positions.
@@ -777,86 +743,6 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
     }
 }
 
-// ---------------------------------------------------------------------------
-// Device: span tile for longer runs of Pauli exponentials
-//
-// The flip masks of a run span D dimensions with a reduced basis b_0..b_{D-1}
-// whose pivots (highest bits) all lie at or above L.  A tile is fixed by the
-// bits of the index outside the pivots and outside the low L bits (the block
-// base J); it holds the 2^(L+D) amplitudes (J | l) ^ c_i, l < 2^L, c_i = XOR
-// of b_k over the bits k of i, in LDS as tile[i * 2^L + l].  Exponential o
-// with flip x = c_v pairs (i, l) with (i ^ v, l), and its sign
-// (-1)^{idx.z} splits into parity(J.z) ^ parity(l.z) ^ parity(i.zb), zb bit k
-// = parity(b_k.z).  Each tile is read and written once per pass; up to
-// kSpanK - L exponentials with independent masks share the pass, against 4 in
-// the register orbit.
-// ---------------------------------------------------------------------------
-
-constexpr int kSpanK = 12;  // tile of at most 2^kSpanK amplitudes (64 KiB of LDS)
-constexpr int kSpanTPB = 256;
-
-struct SpanOp {
-    uint32_t v;   // flip in tile coordinates (0: diagonal exponential)
-    uint32_t zb;  // bit k = parity(b_k & z)
-    uint32_t hv;  // highest bit of v (pairs enumerate i with it clear)
-    uint32_t pad;
-    uint64_t z;
-    double c, sn;
-    double2 ph;
-};
-
-struct SpanDesc {
-    uint32_t L, D, nops, pad;
-    uint32_t piv[kSpanK];  // pivot bits, ascending
-};
-
-__device__ __forceinline__ uint32_t ins0_32(uint32_t k, uint32_t h) {
-    const uint32_t lo = k & ((1u << h) - 1u);
-    return ((k ^ lo) << 1) | lo;
-}
-
-__global__ __launch_bounds__(kSpanTPB) void k_span(double2* __restrict__ s, const SpanDesc* __restrict__ pdesc,
-                                                   const uint64_t* __restrict__ ctab, const SpanOp* __restrict__ ops) {
-    extern __shared__ double2 tile[];
-    const uint32_t L = pdesc->L, D = pdesc->D, nops = pdesc->nops;
-    const uint32_t lmask = (1u << L) - 1u;
-    uint64_t J = (uint64_t)blockIdx.x << L;
-    for (uint32_t k = 0; k < D; ++k) J = ins0(J, pdesc->piv[k]);
-    const uint32_t size = 1u << (L + D);
-    for (uint32_t e = threadIdx.x; e < size; e += kSpanTPB) tile[e] = s[(J | (e & lmask)) ^ ctab[e >> L]];
-    __syncthreads();
-    for (uint32_t o = 0; o < nops; ++o) {
-        const SpanOp op = ops[o];
-        const uint32_t pj = (uint32_t)__popcll(J & op.z) & 1u;
-        const uint32_t zl = (uint32_t)op.z & lmask;
-        const double2 iph = make_double2(-op.ph.y, op.ph.x);  // i * phase
-        if (op.v == 0) {
-            for (uint32_t e = threadIdx.x; e < size; e += kSpanTPB) {
-                const uint32_t sg = pj ^ ((uint32_t)__popc((e & lmask) & zl) & 1u) ^ ((uint32_t)__popc((e >> L) & op.zb) & 1u);
-                const double sn = sg ? -op.sn : op.sn;
-                tile[e] = cmul(make_double2(op.c + sn * iph.x, sn * iph.y), tile[e]);
-            }
-        } else {
-            const uint32_t half = size >> 1;
-            for (uint32_t q = threadIdx.x; q < half; q += kSpanTPB) {
-                const uint32_t l = q & lmask;
-                const uint32_t i1 = ins0_32(q >> L, op.hv), i2 = i1 ^ op.v;
-                const uint32_t e1 = (i1 << L) | l, e2 = (i2 << L) | l;
-                const uint32_t pl = pj ^ ((uint32_t)__popc(l & zl) & 1u);
-                const double s1 = (pl ^ ((uint32_t)__popc(i1 & op.zb) & 1u)) ? -op.sn : op.sn;
-                const double s2 = (pl ^ ((uint32_t)__popc(i2 & op.zb) & 1u)) ? -op.sn : op.sn;
-                const double2 a = tile[e1], b = tile[e2];
-                const double2 ta = cmul(iph, a), tb = cmul(iph, b);
-                // a' = c a + i sin ph (-1)^{idx2.z} b ; b' = c b + i sin ph (-1)^{idx1.z} a
-                tile[e1] = make_double2(op.c * a.x + s2 * tb.x, op.c * a.y + s2 * tb.y);
-                tile[e2] = make_double2(op.c * b.x + s1 * ta.x, op.c * b.y + s1 * ta.y);
-            }
-        }
-        __syncthreads();
-    }
-    for (uint32_t e = threadIdx.x; e < size; e += kSpanTPB) s[(J | (e & lmask)) ^ ctab[e >> L]] = tile[e];
-}
-
 // Reduced GF(2) basis with pivot = highest bit; every pivot bit appears in
 // exactly one basis vector.
 struct Span {
@@ -879,14 +765,13 @@ struct Span {
     }
 };
 
-enum PassKind { P_SINGLE = 0, P_TILE = 1, P_ORBIT = 2, P_SPAN = 3 };
+enum PassKind { P_SINGLE = 0, P_TILE = 1, P_ORBIT = 2 };
 
 struct Pass {
     int kind = P_SINGLE;
     uint64_t bmask = 0;
     std::vector<int> ops;  // indices into the lowered op list
-    Span span;             // P_ORBIT, P_SPAN
-    int span_L = 0;        // P_SPAN: contiguous low bits of a tile
+    Span span;             // P_ORBIT
 };
 
 // Greedy planner.  Ops may move ahead of an earlier op only when their qubit
@@ -953,19 +838,11 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
     while (first < ops.size()) {
         while (first < ops.size() && done[first]) ++first;
         if (first >= ops.size()) break;
-        Pass tile, orbit, span;
+        Pass tile, orbit;
         grow_tile(ops, done, first, K, lowmask, all, tu.max_ops, tile);
         grow_orbit(ops, done, first, all, tu.max_ops, orbit, P_ORBIT, std::max(1, std::min(tu.orbit_dims, kOrbitMax + 2)));
-        // span tiles: low L bits contiguous, up to kSpanK - L dimensions, and
-        // at least 2^8 tiles so the grid fills the chip
-        const int SL = tu.span_low;
-        const int SD = std::min(std::min(kSpanK - SL, n - SL - 8), tu.span_dims);
-        if (SD > kOrbitMax) grow_orbit(ops, done, first, all, tu.max_ops, span, P_SPAN, SD, SL);
-        span.span_L = SL;
         Pass p;
-        if (span.ops.size() > orbit.ops.size() && span.ops.size() >= tile.ops.size() && span.span.b.size() > (size_t)kOrbitMax) {
-            p = std::move(span);
-        } else if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
+        if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
             p = std::move(orbit);
         } else if (tile.ops.size() >= 2) {
             p = std::move(tile);
@@ -1097,60 +974,8 @@ static int launch_orbit_lanes(qfe_state_s* st, int M, const LaneDesc* dd, const 
     return QFE_OK;
 }
 
-// Span descriptor, orbit offsets and per-op data for a finished span pass.
-static void build_span(const Pass& p, const std::vector<HostOp>& ops, SpanDesc& d, std::vector<uint64_t>& ctab,
-                       std::vector<SpanOp>& out) {
-    std::vector<uint64_t> b = p.span.b;
-    std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
-    const int D = (int)b.size();
-    d = SpanDesc{};
-    d.L = (uint32_t)p.span_L;
-    d.D = (uint32_t)D;
-    d.nops = (uint32_t)p.ops.size();
-    for (int k = 0; k < D; ++k) d.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
-    for (int i = 0; i < (1 << D); ++i) {
-        uint64_t c = 0;
-        for (int k = 0; k < D; ++k)
-            if ((i >> k) & 1) c ^= b[k];
-        ctab.push_back(c);
-    }
-    for (int oi : p.ops) {
-        const DevOp& g = ops[oi].op;
-        SpanOp o{};
-        for (int k = 0; k < D; ++k) {
-            if ((g.x >> d.piv[k]) & 1ull) o.v |= 1u << k;
-            if (__builtin_popcountll(b[k] & g.z) & 1) o.zb |= 1u << k;
-        }
-        o.hv = o.v ? (uint32_t)(31 - __builtin_clz(o.v)) : 0u;
-        o.z = g.z;
-        o.c = g.m[0].x;
-        o.sn = g.m[0].y;
-        o.ph = g.m[1];
-        out.push_back(o);
-    }
-}
-
-static int launch_span(qfe_state_s* st, const SpanDesc& hd, const SpanDesc* dd, const uint64_t* ctab, const SpanOp* sops) {
-    Context& cx = ctx();
-    const uint32_t bits = hd.L + hd.D;
-    if (bits > (uint32_t)kSpanK || bits > (uint32_t)st->n) return fail(QFE_ERR_STATE, "internal: span tile out of range");
-    const size_t lds = sizeof(double2) << bits;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_span), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(sizeof(double2) << kSpanK));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(k_span, dim3((unsigned)(st->N >> bits)), dim3(kSpanTPB), lds, cx.stream, st->d, dd, ctab, sops);
-    QFE_HIP(hipGetLastError());
-    cx.counters.fused++;
-    cx.counters.bytes += 32.0 * (double)st->N;
-    return QFE_OK;
-}
-
 template <int KQ>
-static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, const Phase* ph,
-                           hipStream_t st) {
+static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, hipStream_t st) {
     const size_t lds = (size_t)sizeof(double2) << KQ;
     static bool attr_set = false;  // large tiles need the dynamic-LDS limit raised once
     if (!attr_set) {
@@ -1158,27 +983,28 @@ static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, con
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops, ph,
-                       pd.nphases);
+    const uint64_t grid = nblocks < kFusedGrid ? nblocks : kFusedGrid;
+    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)grid), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops, pd.nops,
+                       nblocks);
 }
 
-static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops, const Phase* ph) {
+static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) {
     Context& cx = ctx();
     const int k = (int)pd.k;
     const uint64_t nblocks = st->N >> k;
     switch (k) {
-        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, ph, cx.stream); break;
-        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, ph, cx.stream); break;
+        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, cx.stream); break;
         default: return fail(QFE_ERR_ARG, "internal: fused tile size out of range");
     }
     QFE_HIP(hipGetLastError());
@@ -1237,71 +1063,6 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
     ops.swap(out);
 }
 
-// Split the localized ops [first, first + count) of a tile pass into phases:
-// runs of U1 / DIAG1 / PEXP whose mixing bits fit three tile bits become
-// register phases; any other op is a phase of its own.  Phase op indices are
-// absolute in dops.
-static void build_phases(std::vector<DevOp>& dops, size_t first, uint32_t count, int k, bool reg,
-                         std::vector<Phase>& out) {
-    Phase cur{};
-    uint32_t rmask = 0;
-    bool open = false;
-    auto flush = [&]() {
-        if (!open) return;
-        // pad to three register bits with the lowest free tile bits
-        for (int q = 0; q < k && __builtin_popcount(rmask) < 3; ++q) rmask |= 1u << q;
-        int r = 0;
-        for (int q = 0; q < k && r < 3; ++q)
-            if ((rmask >> q) & 1u) cur.r[r++] = (uint32_t)q;
-        for (uint32_t i = cur.op0; i < cur.op0 + cur.nops; ++i) {
-            DevOp& o = dops[i];
-            const uint32_t m = o.kind == K_U1 ? (1u << o.t) : (o.kind == K_PEXP ? (uint32_t)o.x : 0u);
-            uint32_t v = 0;
-            for (int a = 0; a < 3; ++a)
-                if ((m >> cur.r[a]) & 1u) v |= 1u << a;
-            o.pad = v;
-        }
-        out.push_back(cur);
-        open = false;
-        rmask = 0;
-    };
-    for (uint32_t j = 0; j < count; ++j) {
-        const size_t i = first + j;
-        const DevOp& o = dops[i];
-        uint32_t m = 0;
-        bool eligible = reg && k >= 3;
-        if (o.kind == K_U1)
-            m = 1u << o.t;
-        else if (o.kind == K_DIAG1)
-            m = 0;
-        else if (o.kind == K_PEXP)
-            m = (uint32_t)o.x;
-        else
-            eligible = false;
-        if (eligible && __builtin_popcount(m) > 3) eligible = false;
-        if (!eligible) {
-            flush();
-            Phase single{};
-            single.op0 = (uint32_t)i;
-            single.nops = 1;
-            single.reg = 0;
-            out.push_back(single);
-            continue;
-        }
-        if (open && __builtin_popcount(rmask | m) > 3) flush();
-        if (!open) {
-            cur = Phase{};
-            cur.op0 = (uint32_t)i;
-            cur.nops = 0;
-            cur.reg = 1;
-            open = true;
-        }
-        rmask |= m;
-        cur.nops++;
-    }
-    flush();
-}
-
 int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     if (ops.empty()) return QFE_OK;
     Context& cx = ctx();
@@ -1319,13 +1080,9 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     std::vector<double2> mats;
     std::vector<OrbitDesc> odescs(passes.size());
     std::vector<OrbitOp> oops;
-    std::vector<Phase> phases;
     std::vector<LaneDesc> ldescs;
     std::vector<OrbitOp2> lops;
-    std::vector<SpanDesc> sdescs;
-    std::vector<uint64_t> ctabs;
-    std::vector<SpanOp> sops;
-    std::vector<size_t> span_d(passes.size(), 0), span_c(passes.size(), 0);
+    std::vector<size_t> span_d(passes.size(), 0);
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const Pass& p = passes[pi];
         if (p.kind == P_SINGLE) {
@@ -1346,14 +1103,6 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (p.kind == P_ORBIT) {
             pass_op0[pi] = oops.size();
             build_orbit(p, ops, odescs[pi], oops);
-            continue;
-        }
-        if (p.kind == P_SPAN) {
-            span_d[pi] = sdescs.size();
-            span_c[pi] = ctabs.size();
-            pass_op0[pi] = sops.size();
-            sdescs.emplace_back();
-            build_span(p, ops, sdescs.back(), ctabs, sops);
             continue;
         }
         // extend the tile with the lowest free qubits up to K bits
@@ -1384,9 +1133,6 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         pd.nops = (uint32_t)p.ops.size();
         pass_op0[pi] = dops.size();
         for (int oi : p.ops) dops.push_back(localize(ops[oi].op, b, local_of));
-        pd.phase0 = (uint32_t)phases.size();
-        build_phases(dops, pass_op0[pi], pd.nops, (int)pd.k, cx.tuning.reg_phases != 0, phases);
-        pd.nphases = (uint32_t)phases.size() - pd.phase0;
         pass_pd[pi] = pdescs.size();
         pdescs.push_back(pd);
     }
@@ -1397,21 +1143,13 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     const size_t bytes_mat = mats.size() * sizeof(double2);
     const size_t off_orb = (off_mat + bytes_mat + 255) & ~(size_t)255;
     const size_t bytes_orb = oops.size() * sizeof(OrbitOp);
-    const size_t off_sd = (off_orb + bytes_orb + 255) & ~(size_t)255;
-    const size_t bytes_sd = sdescs.size() * sizeof(SpanDesc);
-    const size_t off_ct = (off_sd + bytes_sd + 255) & ~(size_t)255;
-    const size_t bytes_ct = ctabs.size() * sizeof(uint64_t);
-    const size_t off_so = (off_ct + bytes_ct + 255) & ~(size_t)255;
-    const size_t bytes_so = sops.size() * sizeof(SpanOp);
-    const size_t off_ph = (off_so + bytes_so + 255) & ~(size_t)255;
-    const size_t bytes_ph = phases.size() * sizeof(Phase);
-    const size_t off_ld = (off_ph + bytes_ph + 255) & ~(size_t)255;
+    const size_t off_ld = (off_orb + bytes_orb + 255) & ~(size_t)255;
     const size_t bytes_ld = ldescs.size() * sizeof(LaneDesc);
     const size_t off_lo = (off_ld + bytes_ld + 255) & ~(size_t)255;
     const size_t bytes_lo = lops.size() * sizeof(OrbitOp2);
     const size_t total = off_lo + bytes_lo;
     char* dev = nullptr;
-    if (bytes_ops || bytes_mat || bytes_orb || bytes_sd || bytes_ld) {
+    if (bytes_ops || bytes_mat || bytes_orb || bytes_ld) {
         void* scr = nullptr;
         int rc = scratch_reserve(total, &scr);
         if (rc) return rc;
@@ -1423,10 +1161,6 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         if (bytes_ops) std::memcpy(host + off_ops, dops.data(), bytes_ops);
         if (bytes_mat) std::memcpy(host + off_mat, mats.data(), bytes_mat);
         if (bytes_orb) std::memcpy(host + off_orb, oops.data(), bytes_orb);
-        if (bytes_sd) std::memcpy(host + off_sd, sdescs.data(), bytes_sd);
-        if (bytes_ct) std::memcpy(host + off_ct, ctabs.data(), bytes_ct);
-        if (bytes_so) std::memcpy(host + off_so, sops.data(), bytes_so);
-        if (bytes_ph) std::memcpy(host + off_ph, phases.data(), bytes_ph);
         if (bytes_ld) std::memcpy(host + off_ld, ldescs.data(), bytes_ld);
         if (bytes_lo) std::memcpy(host + off_lo, lops.data(), bytes_lo);
         rc = staging_commit(dev, total);
@@ -1444,14 +1178,9 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         } else if (p.kind == P_ORBIT) {
             const OrbitOp* oo = reinterpret_cast<const OrbitOp*>(dev + off_orb) + pass_op0[pi];
             rc = launch_orbit(st, (int)p.span.b.size(), odescs[pi], oo, (uint32_t)p.ops.size());
-        } else if (p.kind == P_SPAN) {
-            rc = launch_span(st, sdescs[span_d[pi]], reinterpret_cast<const SpanDesc*>(dev + off_sd) + span_d[pi],
-                             reinterpret_cast<const uint64_t*>(dev + off_ct) + span_c[pi],
-                             reinterpret_cast<const SpanOp*>(dev + off_so) + pass_op0[pi]);
         } else {
-            const PassDesc& pd = pdescs[pass_pd[pi]];
-            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops);  // phases index ops of the whole call
-            rc = launch_fused(st, pd, dop, reinterpret_cast<const Phase*>(dev + off_ph) + pd.phase0);
+            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops) + pass_op0[pi];
+            rc = launch_fused(st, pdescs[pass_pd[pi]], dop);
         }
         if (rc) return rc;
     }

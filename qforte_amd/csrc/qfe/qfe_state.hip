@@ -337,24 +337,14 @@ int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass) {
     return QFE_OK;
 }
 
-int qfe_set_pass_tuning(int span_max_dims, int span_low_bits, int register_phases, int orbit_max_dims) {
-    Tuning t = g_ctx.tuning;
-    if (orbit_max_dims >= 0) t.orbit_dims = orbit_max_dims;
-    if (t.orbit_dims < 1 || t.orbit_dims > 6) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: orbit_max_dims must be in [1, 6]");
-    if (span_max_dims >= 0) t.span_dims = span_max_dims;
-    if (span_low_bits >= 0) t.span_low = span_low_bits;
-    if (register_phases >= 0) t.reg_phases = register_phases ? 1 : 0;
-    if (t.span_dims > 12) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: span_max_dims must be in [0, 12]");
-    if (t.span_low > 8) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: span_low_bits must be in [0, 8]");
-    g_ctx.tuning = t;
+int qfe_set_pass_tuning(int orbit_max_dims) {
+    if (orbit_max_dims < 1 || orbit_max_dims > 6) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: orbit_max_dims must be in [1, 6]");
+    g_ctx.tuning.orbit_dims = orbit_max_dims;
     return QFE_OK;
 }
 
-int qfe_get_pass_tuning(int* span_max_dims, int* span_low_bits, int* register_phases, int* orbit_max_dims) {
+int qfe_get_pass_tuning(int* orbit_max_dims) {
     if (orbit_max_dims) *orbit_max_dims = g_ctx.tuning.orbit_dims;
-    if (span_max_dims) *span_max_dims = g_ctx.tuning.span_dims;
-    if (span_low_bits) *span_low_bits = g_ctx.tuning.span_low;
-    if (register_phases) *register_phases = g_ctx.tuning.reg_phases;
     return QFE_OK;
 }
 

@@ -326,17 +326,16 @@ def test_large_state_norm_and_replica(qf, oracle, n):
         assert np.max(np.abs(amps - psi)) < AMP_TOL
 
 
-@pytest.mark.parametrize("tuning,passes", [((11, 4, 24), (8, 4, 1, 4)), ((11, 2, 64), (8, 2, 0, 6)),
-                                           ((11, 6, 9), (0, 4, 0, 5)), ((11, 4, 24), (0, 4, 1, 6))])
-def test_wide_exponential_runs(qf, oracle, tuning, passes):
-    # 18 qubits: runs of exponentials whose flip masks span up to 8
-    # dimensions go to LDS span tiles or to register orbits spread over 2 or 4
-    # lanes; mixed with repeated masks, masks that combine earlier ones, pure
-    # Z strings and masks confined to the low bits
+@pytest.mark.parametrize("tuning,orbit_dims", [((11, 4, 24), 4), ((11, 2, 64), 6), ((11, 6, 9), 5), ((11, 4, 24), 6)])
+def test_wide_exponential_runs(qf, oracle, tuning, orbit_dims):
+    # 18 qubits: runs of exponentials whose flip masks span 5 or 6 dimensions
+    # go to register orbits spread over 2 or 4 lanes; mixed with repeated
+    # masks, masks that combine earlier ones, pure Z strings and masks
+    # confined to the low bits
     n = 18
     qf.set_tuning(*tuning)
     saved = qf.get_pass_tuning()
-    qf.set_pass_tuning(*passes)
+    qf.set_pass_tuning(orbit_dims)
     r = np.random.default_rng(1234)
     seq = []
     for _ in range(90):
@@ -367,10 +366,10 @@ def test_wide_exponential_runs(qf, oracle, tuning, passes):
         expect = oracle.exp_pauli(expect, -1j * theta, term)
     err = np.max(np.abs(qc.get_coeff_vec() - expect))
     if err >= AMP_TOL:
-        qf.set_pass_tuning(*saved)
+        qf.set_pass_tuning(saved)
     assert err < AMP_TOL
     qc.apply_circuit(U.adjoint())
     back = qc.get_coeff_vec()
-    qf.set_pass_tuning(*saved)
+    qf.set_pass_tuning(saved)
     assert npass < len(seq) / 3
     assert np.max(np.abs(back - psi)) < AMP_TOL
