@@ -27,6 +27,11 @@ __device__ __forceinline__ uint64_t ins0(uint64_t k, uint32_t h) {
     uint64_t lo = k & ((1ull << h) - 1ull);
     return ((k ^ lo) << 1) | lo;
 }
+// 32-bit form for indices inside an LDS tile.
+__device__ __forceinline__ uint32_t ins0_32(uint32_t k, uint32_t h) {
+    const uint32_t lo = k & ((1u << h) - 1u);
+    return ((k ^ lo) << 1) | lo;
+}
 // Insert zero bits at two distinct positions lo_b < hi_b.
 __device__ __forceinline__ uint64_t ins00(uint64_t k, uint32_t lo_b, uint32_t hi_b) {
     return ins0(ins0(k, lo_b), hi_b);

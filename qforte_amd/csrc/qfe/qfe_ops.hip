@@ -336,7 +336,6 @@ static int launch_single(qfe_state_s* st, const HostOp& h, double2* dev_mat) {
 
 constexpr int kFusedTPB = 256;
 constexpr int kMaxTileQubits = 13;  // 2^13 * 16 B = 128 KiB of LDS
-constexpr uint64_t kFusedGrid = 1024;  // persistent workgroups of a fused pass (about 4 per CU)
 
 struct PassDesc {
     uint64_t bmask;          // tile qubits (global mask)
@@ -348,32 +347,12 @@ struct PassDesc {
     uint8_t restpos[64];             // global positions of non-tile bits, ascending
 };
 
-// Per-thread prefetch of a tile into registers with compile-time indices only
-// (fold expressions), so the array stays in VGPRs across the tile loop.
-template <uint32_t T, uint32_t... R>
-__device__ __forceinline__ void tile_fetch(double2 (&nxt)[sizeof...(R)], const double2* __restrict__ s, uint64_t base,
-                                           const uint64_t* hioff, uint32_t nlow, uint32_t lowmask,
-                                           std::integer_sequence<uint32_t, R...>) {
-    const uint32_t t = threadIdx.x;
-    ((nxt[R] = (t + R * kFusedTPB < T) ? s[base | hioff[(t + R * kFusedTPB) >> nlow] | ((t + R * kFusedTPB) & lowmask)]
-                                       : make_double2(0.0, 0.0)),
-     ...);
-}
-
-template <uint32_t T, uint32_t... R>
-__device__ __forceinline__ void tile_put(const double2 (&nxt)[sizeof...(R)], double2* tile,
-                                         std::integer_sequence<uint32_t, R...>) {
-    const uint32_t t = threadIdx.x;
-    ((t + R * kFusedTPB < T ? (void)(tile[t + R * kFusedTPB] = nxt[R]) : (void)0), ...);
-}
-
 template <int KQ>
 __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
-                                                     const DevOp* __restrict__ ops, uint32_t nops, uint64_t ntiles) {
+                                                     const DevOp* __restrict__ ops, uint32_t nops) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
     constexpr uint32_t T = 1u << KQ;
-    constexpr uint32_t PER = T >= kFusedTPB ? T / kFusedTPB : 1u;  // amplitudes per thread
     const uint32_t nhi = KQ - nlow;
     const uint64_t himask = bmask & ~((1ull << nlow) - 1ull);
     // tile base: the tile number with a zero bit inserted at every tile qubit
@@ -397,18 +376,8 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
     }
     __syncthreads();
     const uint32_t lowmask = (1u << nlow) - 1u;
-    // Persistent blocks: tiles blockIdx.x, + gridDim.x, ...  The next tile is
-    // loaded into registers while the current one is worked on in LDS.
-    uint64_t tcur = blockIdx.x;
-    uint64_t base = tcur < ntiles ? tile_base(tcur) : 0;
-    double2 nxt[PER];
-    using Seq = std::make_integer_sequence<uint32_t, PER>;
-    if (tcur < ntiles) tile_fetch<T>(nxt, s, base, hioff, nlow, lowmask, Seq{});
-    while (tcur < ntiles) {
-        tile_put<T>(nxt, tile, Seq{});
-        const uint64_t tnext = tcur + gridDim.x;
-        const uint64_t bnext = tnext < ntiles ? tile_base(tnext) : 0;
-        if (tnext < ntiles) tile_fetch<T>(nxt, s, bnext, hioff, nlow, lowmask, Seq{});
+    const uint64_t base = tile_base(blockIdx.x);
+    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) tile[l] = s[base | hioff[l >> nlow] | (l & lowmask)];
     for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
         const DevOp& o = ops[oi];
@@ -418,7 +387,7 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
                 const uint32_t tb = 1u << o.t;
                 if (o.hb) {  // X on the target: swap the pair, no arithmetic
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
-                        const uint32_t l0 = (uint32_t)ins0(p, o.t);
+                        const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
                         const double2 a = tile[l0];
                         tile[l0] = tile[l0 | tb];
@@ -427,13 +396,29 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
                     break;
                 }
                 const double2 m[4] = {o.m[0], o.m[1], o.m[2], o.m[3]};
-                for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
-                    const uint32_t l0 = (uint32_t)ins0(p, o.t);
-                    if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
-                    double2 a = tile[l0], b = tile[l0 | tb];
-                    u2x2(a, b, m);
-                    tile[l0] = a;
-                    tile[l0 | tb] = b;
+                // the pairs of one op are disjoint: all of a thread's pairs are
+                // read before any is written, so the LDS reads overlap
+                constexpr uint32_t PP = (T / 2 + kFusedTPB - 1) / kFusedTPB;
+                uint32_t l0[PP];
+                bool on[PP];
+                double2 a[PP], b[PP];
+#pragma unroll
+                for (uint32_t r = 0; r < PP; ++r) {
+                    const uint32_t p = threadIdx.x + r * kFusedTPB;
+                    l0[r] = ins0_32(p, o.t);
+                    on[r] = p < T / 2 && (l0[r] & (uint32_t)o.cmask) == (uint32_t)o.cval;
+                    if (on[r]) {
+                        a[r] = tile[l0[r]];
+                        b[r] = tile[l0[r] | tb];
+                    }
+                }
+#pragma unroll
+                for (uint32_t r = 0; r < PP; ++r) {
+                    if (on[r]) {
+                        u2x2(a[r], b[r], m);
+                        tile[l0[r]] = a[r];
+                        tile[l0[r] | tb] = b[r];
+                    }
                 }
                 break;
             }
@@ -441,7 +426,7 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
                 if (o.tin) {
                     const uint32_t tb = 1u << o.t;
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
-                        const uint32_t l0 = (uint32_t)ins0(p, o.t);
+                        const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
                         tile[l0] = cmul(o.m[0], tile[l0]);
                         tile[l0 | tb] = cmul(o.m[1], tile[l0 | tb]);
@@ -459,7 +444,7 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
                 const uint32_t lo = o.t < o.c ? o.t : o.c, hi = o.t < o.c ? o.c : o.t;
                 const uint32_t tb = 1u << o.t, cb = 1u << o.c;
                 for (uint32_t p = threadIdx.x; p < T / 4; p += kFusedTPB) {
-                    const uint32_t j = (uint32_t)ins00(p, lo, hi);
+                    const uint32_t j = ins0_32(ins0_32(p, lo), hi);
                     const uint32_t idx[4] = {j, j | tb, j | cb, j | cb | tb};
                     double2 v[4];
 #pragma unroll
@@ -489,7 +474,7 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
                 } else {
                     const uint32_t xl = (uint32_t)o.x;
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
-                        const uint32_t la = (uint32_t)ins0(p, o.hb);
+                        const uint32_t la = ins0_32(p, o.hb);
                         const uint32_t lb = la ^ xl;
                         double2 a = tile[la], b = tile[lb];
                         const bool na = ((__popc(la & zl) & 1) != 0) != bneg;
@@ -507,10 +492,6 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
     }
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) s[base | hioff[l >> nlow] | (l & lowmask)] = tile[l];
-    __syncthreads();  // the tile's LDS is rewritten by the next iteration
-    tcur = tnext;
-    base = bnext;
-    }
 }
 
 // Map a global mask to tile-local bit positions.
@@ -983,9 +964,8 @@ static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, con
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    const uint64_t grid = nblocks < kFusedGrid ? nblocks : kFusedGrid;
-    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)grid), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops, pd.nops,
-                       nblocks);
+    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops,
+                       pd.nops);
 }
 
 static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) {
