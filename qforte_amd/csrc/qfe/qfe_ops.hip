@@ -347,8 +347,66 @@ struct PassDesc {
     uint8_t restpos[64];             // global positions of non-tile bits, ascending
 };
 
+// Apply U1 op `o` along item axis AX (pairs i, i | 2^AX of the 8 amplitudes
+// a thread holds); e[i] are the tile indices (conditions on other bits).
+template <int AX>
+__device__ __forceinline__ void group_axis(double2 (&a)[8], const uint32_t (&e)[8], const DevOp& o, uint64_t base) {
+    if ((base & o.omask) != o.oval) return;  // uniform across the block
+    const uint32_t cm = (uint32_t)o.cmask, cv = (uint32_t)o.cval;
+    const double2 m0 = o.m[0], m1 = o.m[1], m2 = o.m[2], m3 = o.m[3];
+    const bool swap = o.hb != 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i & (1 << AX)) continue;
+        const int j = i | (1 << AX);
+        if ((e[i] & cm) != cv) continue;
+        if (swap) {
+            const double2 t = a[i];
+            a[i] = a[j];
+            a[j] = t;
+        } else {
+            const double2 x = a[i], y = a[j];
+            a[i] = cmix(m0, x, m1, y);
+            a[j] = cmix(m2, x, m3, y);
+        }
+    }
+}
+
+// 2 or 3 consecutive U1 ops with distinct tile targets t0, t1 (, t2): every
+// thread holds the 8 amplitudes spanned by the three bits (a free bit pads a
+// pair) and applies the ops in order, partners at compile-time positions.
+template <uint32_t T>
+__device__ __forceinline__ void fused_group(double2* tile, uint64_t base, const DevOp* __restrict__ g, uint32_t cnt) {
+    const uint32_t t0 = g[0].t, t1 = g[1].t;
+    uint32_t t2 = 0;
+    if (cnt > 2) {
+        t2 = g[2].t;
+    } else {
+        while (t2 == t0 || t2 == t1) ++t2;
+    }
+    // ascending positions for the zero insertions
+    const uint32_t lo = min(t0, min(t1, t2)), hi = max(t0, max(t1, t2));
+    const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
+    const uint32_t b0 = 1u << t0, b1 = 1u << t1, b2 = 1u << t2;
+    for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
+        const uint32_t eb = ins0_32(ins0_32(ins0_32(it, lo), mid), hi);
+        uint32_t e[8];
+        double2 a[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            e[i] = eb | ((i & 1) ? b0 : 0u) | ((i & 2) ? b1 : 0u) | ((i & 4) ? b2 : 0u);
+            a[i] = tile[e[i]];
+        }
+        group_axis<0>(a, e, g[0], base);
+        group_axis<1>(a, e, g[1], base);
+        if (cnt > 2) group_axis<2>(a, e, g[2], base);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) tile[e[i]] = a[i];
+    }
+}
+
 template <int KQ>
-__global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
+__global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <= 11 ? 4 : 1))) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
                                                      const DevOp* __restrict__ ops, uint32_t nops) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
@@ -381,6 +439,14 @@ __global__ __launch_bounds__(kFusedTPB) void k_fused(double2* __restrict__ s, ui
     for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
         const DevOp& o = ops[oi];
+        if (o.kind == K_U1 && o.pad >= 2) {
+            // a group of o.pad one-qubit ops on distinct tile bits: one LDS
+            // round trip for the whole group
+            const uint32_t cnt = o.pad;
+            fused_group<T>(tile, base, ops + oi, cnt);
+            oi += cnt - 1;
+            continue;
+        }
         if ((base & o.omask) != o.oval) continue;  // uniform across the block
         switch (o.kind) {
             case K_U1: {
@@ -1043,6 +1109,33 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
     ops.swap(out);
 }
 
+// Mark runs of 2 or 3 consecutive U1 ops with distinct tile targets (the head
+// op's pad holds the run length): the fused kernel applies each run with one
+// LDS round trip.
+static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count, int k) {
+    if (k < 3) return;
+    uint32_t j = 0;
+    while (j < count) {
+        DevOp& h = dops[first + j];
+        h.pad = 0;
+        if (h.kind != K_U1) {
+            ++j;
+            continue;
+        }
+        uint32_t len = 1;
+        uint32_t used = 1u << h.t;
+        while (len < 3 && j + len < count) {
+            const DevOp& o = dops[first + j + len];
+            if (o.kind != K_U1 || ((used >> o.t) & 1u)) break;
+            used |= 1u << o.t;
+            ++len;
+        }
+        for (uint32_t r = 1; r < len; ++r) dops[first + j + r].pad = 0;
+        h.pad = len >= 2 ? len : 0;
+        j += len;
+    }
+}
+
 int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
     if (ops.empty()) return QFE_OK;
     Context& cx = ctx();
@@ -1113,6 +1206,7 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         pd.nops = (uint32_t)p.ops.size();
         pass_op0[pi] = dops.size();
         for (int oi : p.ops) dops.push_back(localize(ops[oi].op, b, local_of));
+        group_u1_runs(dops, pass_op0[pi], pd.nops, (int)pd.k);
         pass_pd[pi] = pdescs.size();
         pdescs.push_back(pd);
     }
