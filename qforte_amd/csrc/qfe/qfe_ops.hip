@@ -752,7 +752,12 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         off[i] = j ^ hc ^ ((i & 1) ? b0 : 0ull) ^ ((i & 2) ? b1 : 0ull) ^ ((i & 4) ? b2 : 0ull) ^ ((i & 8) ? b3 : 0ull);
-        a[i] = live ? s[off[i]] : make_double2(0.0, 0.0);
+        if (live) {
+            a[i].x = __builtin_nontemporal_load(&s[off[i]].x);
+            a[i].y = __builtin_nontemporal_load(&s[off[i]].y);
+        } else {
+            a[i] = make_double2(0.0, 0.0);
+        }
     }
     const uint32_t hbase = h * 16u;
     for (uint32_t oi = 0; oi < nops; ++oi) {
@@ -786,7 +791,10 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
     }
     if (live) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) s[off[i]] = a[i];
+        for (int i = 0; i < 16; ++i) {
+            __builtin_nontemporal_store(a[i].x, &s[off[i]].x);
+            __builtin_nontemporal_store(a[i].y, &s[off[i]].y);
+        }
     }
 }
 
