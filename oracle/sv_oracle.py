@@ -284,12 +284,14 @@ def apply_pauli(psi: np.ndarray, x: int, z: int, phase: complex) -> np.ndarray:
 
 
 def _popcount(a: np.ndarray) -> np.ndarray:
-    a = a.astype(np.uint64)
-    c = np.zeros(a.shape, dtype=np.int64)
-    while np.any(a):
-        c += (a & np.uint64(1)).astype(np.int64)
-        a = a >> np.uint64(1)
-    return c
+    """Bit count of every element (SWAR reduction on uint64, wrapping multiply)."""
+    x = a.astype(np.uint64)
+    x = x - ((x >> np.uint64(1)) & np.uint64(0x5555555555555555))
+    x = (x & np.uint64(0x3333333333333333)) + ((x >> np.uint64(2)) & np.uint64(0x3333333333333333))
+    x = (x + (x >> np.uint64(4))) & np.uint64(0x0F0F0F0F0F0F0F0F)
+    with np.errstate(over="ignore"):
+        x = (x * np.uint64(0x0101010101010101)) >> np.uint64(56)
+    return x.astype(np.int64)
 
 
 def apply_pauli_circuit(psi: np.ndarray, n: int, term: Sequence[tuple[str, int]]) -> np.ndarray:

@@ -208,3 +208,36 @@ def test_autograd_function_backward_is_the_adjoint_gradient(qf, h4):
         tm[k] -= h
         fd = (prog.energy(list(tp)) - prog.energy(list(tm))) / (2 * h)
         assert abs(2.5 * fd - t.grad[k].item()) < 2.5e-8
+
+
+def test_uccsd_16_qubits_against_dense_oracle(qf, oracle):
+    # the plain UCCSD energy workload: 8 spatial orbitals, 4 alpha + 4 beta
+    # electrons, h symmetric N(0, 0.1) - 2p/8 on the diagonal, g 8-fold
+    # symmetric N(0, 0.01) + 0.5 on g_pppp, JW-mapped, SD pool (360 amplitudes,
+    # 2688 Pauli exponentials), amplitudes N(0, 0.05); FP64 energy against the
+    # dense oracle to 1e-10 Ha, through the full-space program and the sector
+    # template with the Hamiltonian built on the device from the integrals
+    norb = 8
+    h, g = qf.synthetic_integrals(norb, 2024)
+    H = qf.build_sq_hamiltonian(h, g, 0.0).jw_transform()
+    ref = qf.system.hamiltonians.hf_occupation(norb, 4, 4)
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(ref)
+    pool.fill_pool("SD")
+    tops = list(range(len(pool)))
+    assert len(tops) == 360
+    gens, _, _ = qf.ucc_generators(pool, tops)
+    assert sum(len(q.terms()) for q in gens) == 2688
+    t = list(np.random.default_rng(16).normal(0.0, 0.05, len(tops)))
+    exps = [(t[k] * c, term) for k, g_ in enumerate(gens) for c, term in terms_list(g_)]
+    psi = oracle.ucc_state(16, ref, exps)
+    masks = []
+    for c, term in terms_list(H):
+        x, z, ph = oracle.pauli_masks(term)
+        masks.append((c, x, z, ph))
+    e_or = float(np.real(oracle.expval_masks(psi, masks)))  # mask form, pinned against gate by gate
+    prog = qf.compile_ucc_program(16, ref, H, pool, tops)
+    assert abs(prog.energy(t) - e_or) < E_TOL
+    hmap = qf.HamiltonianMap(norb)
+    tpl = qf.compile_ucc_template(16, ref, hmap, pool, tops)
+    assert abs(tpl.energy(t, hmap.pack(0.0, h, g)) - e_or) < E_TOL
