@@ -346,7 +346,8 @@ def main():
     ap.add_argument("--cpu-nqubit", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-roofline", action="store_true", help="skip the dominant-kernel timing")
-    ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops")
+    ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops (skips the autotuner)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default fusion width and tile shape")
     ap.add_argument("--no-serving", action="store_true", help="skip the UCC serving leg")
     ap.add_argument("--no-gradient", action="store_true", help="skip the 20-qubit gradient leg")
     ap.add_argument("--gradient-only", action="store_true", help="only the 20-qubit gradient leg (printed as JSON)")
@@ -372,6 +373,7 @@ def main():
         sys.exit(1)
     if args.tuning:
         qf.set_tuning(*[int(v) for v in args.tuning.split(",")])
+    tune_report = None
 
     if args.serving_only:
         print(json.dumps(serving_leg(qf, W, args.requests, args.plain_per_class)))
@@ -381,6 +383,15 @@ def main():
         return
 
     n = args.nqubit
+    if not args.no_autotune and not args.tuning and n >= 20 and not args.roofline_only:
+        # fusion width and tile shape timed on a 2-qubit-smaller replica of the workload
+        from qforte_amd.engine.autotune import autotune
+
+        m = n - 2
+        rl, _, _, rlc, rU, _ = build_workload(qf, W, m, args.layers, args.nexp, 1)
+        tune_report = autotune(qf, m, [rlc], [rU],
+                               log=lambda msg: print("[bench] " + msg, file=sys.stderr, flush=True))
+        del rl, rlc, rU
     print(f"[bench] building workload n={n} layers={args.layers} exps={args.nexp} terms={args.nterms}",
           file=sys.stderr, flush=True)
     layers, strings, terms, layer_circ, U, H = build_workload(qf, W, n, args.layers, args.nexp, args.nterms)
@@ -489,6 +500,8 @@ def main():
             "nqubit": n,
             "state_bytes": int(S),
             "tuning": list(qf.get_tuning()),
+            "orbit_dims": qf.get_pass_tuning(),
+            "autotune": tune_report,
         },
         "pct_of_peak": round(100.0 * gbps / HBM_PEAK_GBPS, 2),
         "effective_GBps": round(eff_gbps, 1),

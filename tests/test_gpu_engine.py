@@ -373,3 +373,30 @@ def test_wide_exponential_runs(qf, oracle, tuning, orbit_dims):
     qf.set_pass_tuning(saved)
     assert npass < len(seq) / 3
     assert np.max(np.abs(back - psi)) < AMP_TOL
+
+
+def test_autotuner_picks_a_setting_that_stays_exact(qf, oracle):
+    from qforte_amd.engine.autotune import autotune
+
+    n = 14
+    saved_t, saved_o = qf.get_tuning(), qf.get_pass_tuning()
+    layers = W.brickwork_layers(n, 6, seed=5)
+    strings = W.random_pauli_strings(n, 40, seed=6)
+    L = W.to_circuit(qf, layers)
+    U, _ = qf.trotterize(W.exp_operator(qf, strings))
+    rep = autotune(qf, n, [L], [U], tiles=((8, 4, 8), (10, 4, 24)), orbits=(4, 6))
+    try:
+        assert rep["tuning"] in ([8, 4, 8], [10, 4, 24])
+        assert rep["orbit_dims"] in (4, 6)
+        qc = qf.Computer(n)
+        qc.apply_circuit(L)
+        qc.apply_circuit(U)
+        psi = np.zeros(1 << n, complex)
+        psi[0] = 1
+        psi = oracle.apply_circuit(psi, n, [oracle.G(*x) for x in layers])
+        for theta, term in strings:
+            psi = oracle.exp_pauli(psi, -1j * theta, term)
+        assert np.max(np.abs(qc.get_coeff_vec() - psi)) < AMP_TOL
+    finally:
+        qf.set_tuning(*saved_t)
+        qf.set_pass_tuning(saved_o)
