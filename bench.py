@@ -177,6 +177,7 @@ def serving_leg(qf, W, nreq, plain_per_class):
     # the request mix is uniform over the classes
     plain_rate = 1.0 / float(np.mean(list(plain_full.values())))
     sector_rate = 1.0 / float(np.mean(list(plain_sector.values())))
+    grad = serving_gradient_leg(qf, classes, reqs, by_class)
     return {
         "workload": (f"{nreq} UCCSD energy requests, n uniform over {sorted(classes)}, half filled, own seeded "
                      "synthetic integrals per request, amplitudes N(0,0.05)"),
@@ -191,6 +192,57 @@ def serving_leg(qf, W, nreq, plain_per_class):
         "plain_sample_per_class": plain_per_class,
         "max_abs_energy_diff_vs_plain": max_err,
         "sector_dims": {n: c["template"].dim for n, c in classes.items()},
+        "gradient": grad,
+    }
+
+
+def serving_gradient_leg(qf, classes, reqs, by_class):
+    """The same requests as energy + gradient requests: capacity (all queued
+    at once) against the plain per-request gradient paths on the sample."""
+    import numpy as np
+
+    eng = qf.ServingEngine(hbm_budget_bytes=64 << 30, max_active=8192)
+    t0 = time.perf_counter()
+    ids = [eng.submit(classes[n]["template"], t, inputs, arrival_time=t0, gradient=True) for n, inputs, t in reqs]
+    eng.drain()
+    cap_s = time.perf_counter() - t0
+    capacity = len(reqs) / cap_s
+    print(f"[bench] gradient serving capacity {capacity:.0f} evals/s ({cap_s:.3f} s for {len(reqs)})",
+          file=sys.stderr, flush=True)
+    plain_full, plain_sector, max_err = {}, {}, 0.0
+    for n, idx in by_class.items():
+        c = classes[n]
+        gens, pidx, scale = qf.ucc_generators(c["pool"], c["tops"])
+        tf, ts = [], []
+        for i in idx:
+            _, inputs, t = reqs[i]
+            prog = qf.UCCProgram(n, qf.ref_to_basis_idx(c["ref"]), gens, pidx, scale, len(c["tops"]),
+                                 c["map"].operator(inputs))
+            a = time.perf_counter()
+            e_full, g_full = prog.gradient(t)
+            b = time.perf_counter()
+            e_sec, g_sec = c["template"].gradient(t, inputs)
+            d = time.perf_counter()
+            tf.append(b - a)
+            ts.append(d - b)
+            g_srv = np.array(eng.gradients[ids[i]])
+            max_err = max(max_err, float(np.max(np.abs(g_srv - np.array(g_full)))),
+                          float(np.max(np.abs(g_srv - np.array(g_sec)))), abs(eng.results[ids[i]] - e_full))
+            del prog
+        plain_full[n], plain_sector[n] = float(np.mean(tf)), float(np.mean(ts))
+        print(f"[bench] plain gradient n={n}: full-space {1e3 * plain_full[n]:.3f} ms, sector one-request "
+              f"{1e3 * plain_sector[n]:.3f} ms", file=sys.stderr, flush=True)
+    plain_rate = 1.0 / float(np.mean(list(plain_full.values())))
+    sector_rate = 1.0 / float(np.mean(list(plain_sector.values())))
+    return {
+        "capacity_evals_per_s": round(capacity, 1),
+        "plain_full_space_evals_per_s": round(plain_rate, 2),
+        "plain_sector_one_request_evals_per_s": round(sector_rate, 2),
+        "speedup_vs_plain": round(capacity / plain_rate, 1),
+        "speedup_vs_plain_sector": round(capacity / sector_rate, 1),
+        "plain_ms_per_gradient_by_n": {n: round(1e3 * v, 3) for n, v in plain_full.items()},
+        "plain_sector_ms_per_gradient_by_n": {n: round(1e3 * v, 3) for n, v in plain_sector.items()},
+        "max_abs_diff_vs_plain": max_err,
     }
 
 

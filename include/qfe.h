@@ -234,7 +234,7 @@ int qfe_template_gradient(qfe_template t, const double* params, const double* in
 int qfe_template_pool_gradients(qfe_template t, const double* params, const double* inputs, const qfe_pauli* pool_terms,
                                 const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 
-/* ---- serving: continuous batching of UCC energy requests ----------------
+/* ---- serving: continuous batching of UCC energy and gradient requests ----
  * No reference counterpart: the reference evaluates one energy per Python
  * call.  Requests (a template, amplitudes and Hamiltonian inputs) queue on
  * the host; each engine step admits queued requests into a launch while the
@@ -246,6 +246,15 @@ int qfe_batch_create(uint64_t hbm_budget_bytes, int max_active, qfe_batch* out);
 int qfe_batch_destroy(qfe_batch b);
 /* Queue a request; params and inputs are copied.  The template must outlive it. */
 int qfe_batch_submit(qfe_batch b, qfe_template t, const double* params, const double* inputs, uint64_t* request_id);
+/* Queue an energy + adjoint gradient request (the qfe_template_gradient
+ * computation; uccvqeabc.py:122-251 for the reference's per-call form).  Its
+ * energy is returned by step/poll like any other; once it has been returned
+ * the gradient (nparam values) is fetched once with qfe_batch_take_gradient.
+ * A launch holds requests of one kind, taken from the kind whose oldest
+ * queued request arrived first. */
+int qfe_batch_submit_gradient(qfe_batch b, qfe_template t, const double* params, const double* inputs,
+                              uint64_t* request_id);
+int qfe_batch_take_gradient(qfe_batch b, uint64_t request_id, double* grad, uint64_t nparam);
 /* Launch what can be admitted; return completed results (ids, energies, up to
  * cap).  Blocks for the oldest launch only when nothing else can progress. */
 int qfe_batch_step(qfe_batch b, uint64_t* done_ids, double* done_energies, uint64_t cap, uint64_t* ndone);

@@ -244,12 +244,21 @@ class BatchEngine {
     ~BatchEngine() {
         if (b_) qfe_batch_destroy(b_);
     }
-    uint64_t submit(std::shared_ptr<SectorTemplate> t, const std::vector<double>& amps, const std::vector<double>& inputs) {
+    uint64_t submit(std::shared_ptr<SectorTemplate> t, const std::vector<double>& amps, const std::vector<double>& inputs,
+                    bool gradient) {
         t->need(amps, inputs);
         uint64_t id = 0;
-        check(qfe_batch_submit(b_, t->handle(), amps.data(), inputs.data(), &id), "BatchEngine.submit");
+        if (gradient)
+            check(qfe_batch_submit_gradient(b_, t->handle(), amps.data(), inputs.data(), &id), "BatchEngine.submit");
+        else
+            check(qfe_batch_submit(b_, t->handle(), amps.data(), inputs.data(), &id), "BatchEngine.submit");
         keep_.push_back(t);
         return id;
+    }
+    std::vector<double> take_gradient(uint64_t id, size_t nparam) {
+        std::vector<double> g(nparam);
+        check(qfe_batch_take_gradient(b_, id, g.data(), nparam), "BatchEngine.take_gradient");
+        return g;
     }
     std::vector<std::pair<uint64_t, double>> step(size_t cap) { return run(cap, true); }
     std::vector<std::pair<uint64_t, double>> poll(size_t cap) { return run(cap, false); }
@@ -621,7 +630,9 @@ PYBIND11_MODULE(_core, m) {
 
     py::class_<BatchEngine>(m, "BatchEngine")
         .def(py::init<uint64_t, int>(), "hbm_budget_bytes"_a, "max_active"_a)
-        .def("submit", &BatchEngine::submit, "template"_a, "amplitudes"_a, "inputs"_a = std::vector<double>())
+        .def("submit", &BatchEngine::submit, "template"_a, "amplitudes"_a, "inputs"_a = std::vector<double>(),
+             "gradient"_a = false)
+        .def("take_gradient", &BatchEngine::take_gradient, "request_id"_a, "nparam"_a)
         .def("step", &BatchEngine::step, "cap"_a = 4096)
         .def("poll", &BatchEngine::poll, "cap"_a = 4096)
         .def("pending", &BatchEngine::pending);

@@ -39,6 +39,7 @@
 #include <deque>
 #include <map>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "qfe_common.cuh"
@@ -486,6 +487,18 @@ int launch_ansatz(int tier, unsigned nblocks, size_t lds, hipStream_t st, const 
     return QFE_OK;
 }
 
+// Reverse sweep launch shape follows the ansatz tier (256 / 512 / 1024 threads).
+int launch_reverse(int tier, unsigned nblocks, hipStream_t st, const SlotDesc* d) {
+    if (tier == 0)
+        hipLaunchKernelGGL((k_sec_reverse<256>), dim3(nblocks), dim3(256), 0, st, d);
+    else if (tier == 1)
+        hipLaunchKernelGGL((k_sec_reverse<512>), dim3(nblocks), dim3(512), 0, st, d);
+    else
+        hipLaunchKernelGGL((k_sec_reverse<1024>), dim3(nblocks), dim3(1024), 0, st, d);
+    QFE_HIP(hipGetLastError());
+    return QFE_OK;
+}
+
 // ---- host-side construction -------------------------------------------------------------
 
 inline uint32_t compress(uint64_t v, int parity, int norb) {
@@ -678,7 +691,7 @@ struct qfe_template_s {
     const uint64_t* bfull = nullptr;
     // per-request device bytes: psi, coef, cs, part (energy); + sig, gacc (gradient)
     size_t off_psi = 0, off_coef = 0, off_cs = 0, off_part = 0, off_sig = 0, off_gacc = 0;
-    size_t work_bytes = 0, grad_bytes = 0, io_count = 0, request_bytes = 0;
+    size_t work_bytes = 0, grad_bytes = 0, io_count = 0, request_bytes = 0, grad_request_bytes = 0;
     void* arena = nullptr;  // single-request arena: [desc][out][io][prefix][work incl. gradient parts]
 };
 
@@ -927,6 +940,7 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     t->grad_bytes = o;
     t->io_count = nparam + ninputs;
     t->request_bytes = t->work_bytes + align256(t->io_count * 8 + 8) + sizeof(SlotDesc) + 16;
+    t->grad_request_bytes = t->grad_bytes + align256(t->io_count * 8 + 8) + sizeof(SlotDesc) + 16 + ngen * 8;
     return QFE_OK;
 }
 
@@ -960,6 +974,21 @@ int launch_energy(const std::vector<qfe_template>& ts, int tier, size_t lds, con
     hipLaunchKernelGGL(k_sec_sum, dim3(k), dim3(256), 0, st, ddesc);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
+}
+
+// Energy and adjoint gradient of k requests of one tier: prologue, ansatz,
+// sigma = H psi (grid y = request), reverse sweep.
+int launch_gradient(const std::vector<qfe_template>& ts, int tier, size_t lds, const SlotDesc* ddesc, hipStream_t st) {
+    const unsigned k = (unsigned)ts.size();
+    uint64_t chunks = 1;
+    for (const auto& t : ts) chunks = std::max<uint64_t>(chunks, (t->D + 255) / 256);
+    hipLaunchKernelGGL(k_sec_prologue, dim3(k), dim3(256), 0, st, ddesc, 1);
+    QFE_HIP(hipGetLastError());
+    int rc = launch_ansatz(tier, k, lds, st, ddesc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sec_hpsi, dim3((unsigned)chunks, k), dim3(256), 0, st, ddesc);
+    QFE_HIP(hipGetLastError());
+    return launch_reverse(tier, k, st, ddesc);
 }
 
 // Single request on the library stream.  Arena: [desc 256][out 256][prefix 256][io][work + gradient parts].
@@ -1002,7 +1031,13 @@ int fetch_doubles(const double* dev, size_t count, double* host) {
     return QFE_OK;
 }
 
-// psi = U|ref> and sigma = H psi for one request (gradient paths).
+// grad[p] = sum over generators k with parameter p of 2 scale_k <sigma_k|K_k|psi_k>.
+void gradient_from_overlaps(qfe_template t, const double* acc, double* grad) {
+    for (uint64_t i = 0; i < t->nparam; ++i) grad[i] = 0.0;
+    for (uint64_t k = 0; k < t->ngen; ++k) grad[t->gparam[k]] += 2.0 * t->gscale[k] * acc[k];
+}
+
+// psi = U|ref> and sigma = H psi for one request (pool gradients).
 int single_psi_sigma(qfe_template t, const double* params, const double* inputs, Single& S) {
     int rc = single_setup(t, params, inputs, true, S);
     if (rc) return rc;
@@ -1039,7 +1074,10 @@ struct Lane {
     double* out = nullptr;
     size_t out_cap = 0;
     std::vector<uint64_t> ids;  // in slot order
+    std::vector<qfe_template> ts;
+    std::vector<size_t> goff;  // gradient launches: each request's first overlap after the k energies
     bool busy = false;
+    bool grad = false;
     int tier = 0;
     uint64_t seq = 0;  // launch order
 };
@@ -1052,9 +1090,11 @@ struct qfe_batch_s {
     uint64_t next_id = 1;
     uint64_t active = 0;
     uint64_t seq = 0;
-    std::deque<Request> queue[3];  // one queue per ansatz tier
+    std::deque<Request> queue[3];   // energy requests, one queue per ansatz tier
+    std::deque<Request> gqueue[3];  // energy + gradient requests
     std::vector<Lane> lanes;
     std::deque<std::pair<uint64_t, double>> done;
+    std::unordered_map<uint64_t, std::vector<double>> grads;  // completed gradients not yet taken
 };
 
 namespace {
@@ -1085,14 +1125,22 @@ int grow_pinned(void*& p, size_t& cap, size_t need) {
     return QFE_OK;
 }
 
+// Largest launch of gradient requests (the sigma kernel's grid y is the request).
+constexpr size_t kMaxGradLaunch = 65535;
+
 int lane_launch(qfe_batch b, Lane& L) {
-    // admit from the front of the tier's queue while the footprint fits this lane's share
+    // a launch holds one kind: the kind whose oldest queued request arrived first
+    std::deque<Request>& qe = b->queue[L.tier];
+    std::deque<Request>& qg = b->gqueue[L.tier];
+    const bool grad = qe.empty() || (!qg.empty() && qg.front().id < qe.front().id);
+    std::deque<Request>& queue = grad ? qg : qe;
+    // admit from the front of the queue while the footprint fits this lane's share
     const uint64_t share = b->budget / kLanes;
-    std::deque<Request>& queue = b->queue[L.tier];
     std::vector<Request> take;
     uint64_t foot = 0;
-    while (!queue.empty() && (int64_t)(b->active + take.size()) < b->max_active) {
-        const uint64_t f = queue.front().t->request_bytes;
+    while (!queue.empty() && (int64_t)(b->active + take.size()) < b->max_active &&
+           (!grad || take.size() < kMaxGradLaunch)) {
+        const uint64_t f = grad ? queue.front().t->grad_request_bytes : queue.front().t->request_bytes;
         if (!take.empty() && foot + f > share) break;
         foot += f;
         take.push_back(std::move(queue.front()));
@@ -1102,19 +1150,22 @@ int lane_launch(qfe_batch b, Lane& L) {
     // heaviest requests first, so their workgroups start first
     std::stable_sort(take.begin(), take.end(), [](const Request& x, const Request& y) { return x.t->cost > y.t->cost; });
     const size_t k = take.size();
-    size_t io_total = 0, work_total = 0, lds = 0;
+    size_t io_total = 0, work_total = 0, lds = 0, gtotal = 0;
     uint64_t items = 0;
     std::vector<qfe_template> ts(k);
     for (size_t i = 0; i < k; ++i) {
         ts[i] = take[i].t;
         io_total += ts[i]->io_count;
-        work_total += ts[i]->work_bytes;
+        work_total += grad ? ts[i]->grad_bytes : ts[i]->work_bytes;
         lds = std::max(lds, ts[i]->lds_bytes);
         items += ts[i]->nitems;
+        if (grad) gtotal += ts[i]->ngen;
     }
     if (items >= (1ull << 31)) return fail(QFE_ERR_ARG, "qfe_batch: too many work items in one launch");
+    // results: k energies, then the generator overlaps of every gradient request
+    const size_t nres = k + gtotal;
     const size_t o_out = align256(k * sizeof(SlotDesc));
-    const size_t o_pre = o_out + align256(k * 8);
+    const size_t o_pre = o_out + align256(nres * 8);
     const size_t o_io = o_pre + align256((k + 1) * 4);
     const size_t head = o_io + align256(io_total * 8 + 8);
     const size_t need = head + work_total;
@@ -1123,7 +1174,7 @@ int lane_launch(qfe_batch b, Lane& L) {
     if (!rc) {
         void* p = L.out;
         size_t cap = L.out_cap * 8;
-        rc = grow_pinned(p, cap, k * 8 + 8);
+        rc = grow_pinned(p, cap, nres * 8 + 8);
         L.out = (double*)p;
         L.out_cap = cap / 8;
     }
@@ -1132,29 +1183,40 @@ int lane_launch(qfe_batch b, Lane& L) {
     char* up = (char*)L.up;
     double* dout = (double*)(dev + o_out);
     uint32_t* hpre = (uint32_t*)(up + o_pre);
-    size_t io_pos = 0, work_pos = head;
+    size_t io_pos = 0, work_pos = head, gpos = k;
     uint32_t acc_items = 0;
     L.ids.clear();
+    L.goff.clear();
     for (size_t i = 0; i < k; ++i) {
         qfe_template t = ts[i];
         const double* dio = (const double*)(dev + o_io) + io_pos;
         double* hio = (double*)(up + o_io) + io_pos;
-        SlotDesc d = make_slot(t, dev + work_pos, dio, dout + i, false);
+        SlotDesc d = make_slot(t, dev + work_pos, dio, dout + i, grad);
+        if (grad) {
+            d.gacc = dout + gpos;
+            L.goff.push_back(gpos);
+            gpos += t->ngen;
+        }
         std::memcpy(up + i * sizeof(SlotDesc), &d, sizeof(d));
         if (t->nparam) std::memcpy(hio, take[i].params.data(), t->nparam * 8);
         if (t->ninputs) std::memcpy(hio + t->nparam, take[i].inputs.data(), t->ninputs * 8);
         hpre[i] = acc_items;
         acc_items += t->nitems;
         io_pos += t->io_count;
-        work_pos += t->work_bytes;
+        work_pos += grad ? t->grad_bytes : t->work_bytes;
         L.ids.push_back(take[i].id);
     }
     hpre[k] = acc_items;
     QFE_HIP(hipMemcpyAsync(dev, up, head, hipMemcpyHostToDevice, L.stream));
-    rc = launch_energy(ts, L.tier, lds, (const SlotDesc*)dev, (const uint32_t*)(dev + o_pre), acc_items, L.stream);
+    if (grad)
+        rc = launch_gradient(ts, L.tier, lds, (const SlotDesc*)dev, L.stream);
+    else
+        rc = launch_energy(ts, L.tier, lds, (const SlotDesc*)dev, (const uint32_t*)(dev + o_pre), acc_items, L.stream);
     if (rc) return rc;
-    QFE_HIP(hipMemcpyAsync(L.out, dout, k * 8, hipMemcpyDeviceToHost, L.stream));
+    QFE_HIP(hipMemcpyAsync(L.out, dout, nres * 8, hipMemcpyDeviceToHost, L.stream));
     QFE_HIP(hipEventRecord(L.done, L.stream));
+    L.ts = std::move(ts);
+    L.grad = grad;
     L.busy = true;
     L.seq = ++b->seq;
     b->active += k;
@@ -1163,6 +1225,12 @@ int lane_launch(qfe_batch b, Lane& L) {
 
 void retire(qfe_batch b, Lane& L) {
     for (size_t i = 0; i < L.ids.size(); ++i) b->done.emplace_back(L.ids[i], L.out[i]);
+    if (L.grad)
+        for (size_t i = 0; i < L.ids.size(); ++i) {
+            std::vector<double> g(L.ts[i]->nparam);
+            gradient_from_overlaps(L.ts[i], L.out + L.goff[i], g.data());
+            b->grads[L.ids[i]] = std::move(g);
+        }
     b->active -= L.ids.size();
     L.ids.clear();
     L.busy = false;
@@ -1190,6 +1258,15 @@ int collect(qfe_batch b, bool wait_oldest) {
     return QFE_OK;
 }
 
+Request make_request(qfe_batch b, qfe_template t, const double* params, const double* inputs) {
+    Request r;
+    r.id = b->next_id++;
+    r.t = t;
+    r.params.assign(params, params + t->nparam);
+    if (t->ninputs) r.inputs.assign(inputs, inputs + t->ninputs);
+    return r;
+}
+
 bool any_busy(qfe_batch b) {
     for (const auto& L : b->lanes)
         if (L.busy) return true;
@@ -1198,7 +1275,7 @@ bool any_busy(qfe_batch b) {
 
 int fill_lanes(qfe_batch b) {
     for (auto& L : b->lanes) {
-        if (L.busy || b->queue[L.tier].empty()) continue;
+        if (L.busy || (b->queue[L.tier].empty() && b->gqueue[L.tier].empty())) continue;
         int rc = lane_launch(b, L);
         if (rc) return rc;
     }
@@ -1293,16 +1370,14 @@ int qfe_template_gradient(qfe_template t, const double* params, const double* in
     if (!t || !grad || (t->nparam && !params) || (t->ninputs && !inputs))
         return fail(QFE_ERR_ARG, "qfe_template_gradient: null argument");
     Single S;
-    int rc = single_psi_sigma(t, params, inputs, S);
+    int rc = single_setup(t, params, inputs, true, S);
+    if (!rc) rc = launch_gradient({t}, t->tier, t->lds_bytes, S.ddesc, ctx().stream);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_sec_reverse<1024>), dim3(1), dim3(1024), 0, ctx().stream, S.ddesc);
-    QFE_HIP(hipGetLastError());
     std::vector<double> acc(t->ngen + 1);
     rc = fetch_doubles((const double*)(S.work + t->off_gacc), t->ngen, acc.data());
     if (!rc) rc = fetch_doubles(S.dout, 1, &acc[t->ngen]);
     if (rc) return rc;
-    for (uint64_t i = 0; i < t->nparam; ++i) grad[i] = 0.0;
-    for (uint64_t k = 0; k < t->ngen; ++k) grad[t->gparam[k]] += 2.0 * t->gscale[k] * acc[k];
+    gradient_from_overlaps(t, acc.data(), grad);
     if (energy) *energy = acc[t->ngen];
     return QFE_OK;
 }
@@ -1403,13 +1478,31 @@ int qfe_batch_submit(qfe_batch b, qfe_template t, const double* params, const do
     if ((t->nparam && !params) || (t->ninputs && !inputs)) return fail(QFE_ERR_ARG, "qfe_batch_submit: null parameters");
     if (t->request_bytes > b->budget / kLanes)
         return fail(QFE_ERR_ARG, "qfe_batch_submit: one request exceeds the per-launch share of the HBM budget");
-    Request r;
-    r.id = b->next_id++;
-    r.t = t;
-    r.params.assign(params, params + t->nparam);
-    if (t->ninputs) r.inputs.assign(inputs, inputs + t->ninputs);
-    *request_id = r.id;
-    b->queue[t->tier].push_back(std::move(r));
+    b->queue[t->tier].push_back(make_request(b, t, params, inputs));
+    *request_id = b->queue[t->tier].back().id;
+    return QFE_OK;
+}
+
+int qfe_batch_submit_gradient(qfe_batch b, qfe_template t, const double* params, const double* inputs,
+                              uint64_t* request_id) {
+    if (!b || !t || !request_id) return fail(QFE_ERR_ARG, "qfe_batch_submit_gradient: invalid argument");
+    if ((t->nparam && !params) || (t->ninputs && !inputs))
+        return fail(QFE_ERR_ARG, "qfe_batch_submit_gradient: null parameters");
+    if (t->grad_request_bytes > b->budget / kLanes)
+        return fail(QFE_ERR_ARG, "qfe_batch_submit_gradient: one request exceeds the per-launch share of the HBM budget");
+    b->gqueue[t->tier].push_back(make_request(b, t, params, inputs));
+    *request_id = b->gqueue[t->tier].back().id;
+    return QFE_OK;
+}
+
+int qfe_batch_take_gradient(qfe_batch b, uint64_t request_id, double* grad, uint64_t nparam) {
+    if (!b || (nparam && !grad)) return fail(QFE_ERR_ARG, "qfe_batch_take_gradient: invalid argument");
+    auto it = b->grads.find(request_id);
+    if (it == b->grads.end())
+        return fail(QFE_ERR_ARG, "qfe_batch_take_gradient: no completed gradient request with this id");
+    if (it->second.size() != nparam) return fail(QFE_ERR_ARG, "qfe_batch_take_gradient: wrong gradient length");
+    std::copy(it->second.begin(), it->second.end(), grad);
+    b->grads.erase(it);
     return QFE_OK;
 }
 
@@ -1435,7 +1528,10 @@ int qfe_batch_step(qfe_batch b, uint64_t* done_ids, double* done_energies, uint6
 
 int qfe_batch_pending(qfe_batch b, uint64_t* queued, uint64_t* active) {
     if (!b) return fail(QFE_ERR_ARG, "qfe_batch_pending: invalid handle");
-    if (queued) *queued = b->queue[0].size() + b->queue[1].size() + b->queue[2].size();
+    if (queued) {
+        *queued = 0;
+        for (int i = 0; i < 3; ++i) *queued += b->queue[i].size() + b->gqueue[i].size();
+    }
     if (active) *active = b->active + b->done.size();
     return QFE_OK;
 }

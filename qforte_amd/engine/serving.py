@@ -1,4 +1,4 @@
-"""Continuous-batching serving loop for UCC energy requests.
+"""Continuous-batching serving loop for UCC energy and gradient requests.
 
 A request is a ``SectorTemplate`` (one size class: ansatz and Hamiltonian
 structure), its amplitudes and its Hamiltonian inputs (integrals packed by
@@ -7,7 +7,10 @@ on the host; every ``step``/``poll`` launches what can be admitted onto one of
 a few streams (one workgroup per request, requests of any size class in the
 same launch, bounded by the HBM budget), so work that arrives while other
 launches run is admitted into the next launch and launches overlap on the
-device.  This module adds arrival-time bookkeeping and latency statistics.
+device.  A gradient request (``gradient=True``) returns the energy and the
+adjoint gradient of the amplitudes (``SectorTemplate.gradient``); it is
+launched with other gradient requests of its size class.  This module adds
+arrival-time bookkeeping and latency statistics.
 """
 
 import time
@@ -23,16 +26,23 @@ class ServingEngine:
         self._arrival = {}
         self.latencies = {}
         self.results = {}
+        self.gradients = {}
+        self._grad_len = {}
 
-    def submit(self, template, amplitudes, inputs=(), arrival_time=None):
-        rid = self._eng.submit(template, list(amplitudes), list(inputs))
+    def submit(self, template, amplitudes, inputs=(), arrival_time=None, gradient=False):
+        rid = self._eng.submit(template, list(amplitudes), list(inputs), bool(gradient))
         self._arrival[rid] = time.perf_counter() if arrival_time is None else arrival_time
+        if gradient:
+            self._grad_len[rid] = template.nparam
         return rid
 
     def _record(self, done):
         now = time.perf_counter()
         for rid, e in done:
             self.results[rid] = e
+            n = self._grad_len.pop(rid, None)
+            if n is not None:
+                self.gradients[rid] = self._eng.take_gradient(rid, n)
             self.latencies[rid] = now - self._arrival.pop(rid)
         return done
 

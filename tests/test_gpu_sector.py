@@ -121,6 +121,39 @@ def test_serving_results_are_the_same_alone_and_in_the_mix(qf):
     assert 0 < p["p50"] <= p["p99"]
 
 
+def test_serving_gradients_match_the_template_alone(qf):
+    """Gradient requests interleaved with energy requests of every size class:
+    each returns the energy and gradient of ``SectorTemplate.gradient`` alone."""
+    cases = []
+    for norb, na, nb in [(4, 2, 2), (6, 3, 3), (8, 4, 4), (10, 5, 5)]:
+        _, _, ref = synthetic_case(qf, norb, na, nb, 0)
+        pool = sd_pool(qf, ref)
+        hmap = qf.HamiltonianMap(norb)
+        tpl = qf.compile_ucc_template(2 * norb, ref, hmap, pool, list(range(len(pool))))
+        cases.append((norb, tpl, hmap))
+    rng = np.random.default_rng(9)
+    reqs = []
+    for k in range(40):
+        norb, tpl, hmap = cases[k % len(cases)]
+        h, g = qf.synthetic_integrals(norb, 2000 + k)
+        reqs.append((tpl, list(rng.normal(0.0, 0.05, tpl.nparam)), hmap.pack(0.0, h, g), k % 3 != 0))
+    eng = qf.ServingEngine(hbm_budget_bytes=1 << 30, max_active=4096)
+    ids = []
+    for wave in range(2):
+        for tpl, t, inp, want_grad in reqs[wave * 20:(wave + 1) * 20]:
+            ids.append(eng.submit(tpl, t, inp, gradient=want_grad))
+        eng.poll()
+    eng.drain()
+    assert set(eng.gradients) == {i for i, r in zip(ids, reqs) if r[3]}
+    for rid, (tpl, t, inp, want_grad) in zip(ids, reqs):
+        if want_grad:
+            e, gr = tpl.gradient(t, inp)
+            assert abs(eng.results[rid] - e) < 1e-12
+            assert np.max(np.abs(np.array(eng.gradients[rid]) - np.array(gr))) < 1e-12
+        else:
+            assert abs(eng.results[rid] - tpl.energy(t, inp)) < 1e-12
+
+
 def test_serving_respects_the_budget(qf):
     _, _, ref = synthetic_case(qf, 6, 3, 3, 0)
     pool = sd_pool(qf, ref)
