@@ -151,11 +151,11 @@ def serving_leg(qf, W, nreq, plain_per_class):
 
     # plain paths on a sample: per class, the first plain_per_class requests
     by_class = {n: [i for i, r in enumerate(reqs) if r[0] == n][:plain_per_class] for n in classes}
-    plain_full, plain_sector, create_full, max_err = {}, {}, {}, 0.0
+    plain_full, plain_sector, create_full, plain_graph, capture, max_err = {}, {}, {}, {}, {}, 0.0
     for n, idx in by_class.items():
         c = classes[n]
         gens, pidx, scale = qf.ucc_generators(c["pool"], c["tops"])
-        tf, ts, tc = [], [], []
+        tf, ts, tc, tg, tcap = [], [], [], [], []
         for i in idx:
             _, inputs, t = reqs[i]
             a = time.perf_counter()
@@ -166,14 +166,22 @@ def serving_leg(qf, W, nreq, plain_per_class):
             d = time.perf_counter()
             e_sec = c["template"].energy(t, inputs)
             f = time.perf_counter()
+            prog.energy_graph(t)  # plan with device-side angles and capture
+            g = time.perf_counter()
+            e_graph = prog.energy_graph(t)  # replay
+            h = time.perf_counter()
             tc.append(b - a)
             tf.append(d - b)
             ts.append(f - d)
-            max_err = max(max_err, abs(e_full - batched[i]), abs(e_sec - batched[i]))
+            tcap.append(g - f)
+            tg.append(h - g)
+            max_err = max(max_err, abs(e_full - batched[i]), abs(e_sec - batched[i]), abs(e_graph - batched[i]))
             del prog
         plain_full[n], plain_sector[n], create_full[n] = float(np.mean(tf)), float(np.mean(ts)), float(np.mean(tc))
+        plain_graph[n], capture[n] = float(np.mean(tg)), float(np.mean(tcap))
         print(f"[bench] plain n={n}: full-space {1e3 * plain_full[n]:.3f} ms/eval (+{1e3 * create_full[n]:.1f} ms "
-              f"program build), sector one-launch {1e3 * plain_sector[n]:.3f} ms/eval", file=sys.stderr, flush=True)
+              f"program build), graph replay {1e3 * plain_graph[n]:.3f} ms (capture {1e3 * capture[n]:.1f} ms), "
+              f"sector one-launch {1e3 * plain_sector[n]:.3f} ms/eval", file=sys.stderr, flush=True)
     # the request mix is uniform over the classes
     plain_rate = 1.0 / float(np.mean(list(plain_full.values())))
     sector_rate = 1.0 / float(np.mean(list(plain_sector.values())))
@@ -188,6 +196,8 @@ def serving_leg(qf, W, nreq, plain_per_class):
         "speedup_vs_plain": round(capacity / plain_rate, 1),
         "speedup_vs_plain_sector": round(capacity / sector_rate, 1),
         "plain_ms_per_eval_by_n": {n: round(1e3 * v, 3) for n, v in plain_full.items()},
+        "plain_graph_replay_ms_by_n": {n: round(1e3 * v, 3) for n, v in plain_graph.items()},
+        "plain_graph_capture_ms_by_n": {n: round(1e3 * v, 2) for n, v in capture.items()},
         "plain_program_build_ms_by_n": {n: round(1e3 * v, 2) for n, v in create_full.items()},
         "plain_sample_per_class": plain_per_class,
         "max_abs_energy_diff_vs_plain": max_err,

@@ -179,6 +179,14 @@ int qfe_program_destroy(qfe_program p);
 int qfe_program_energy(qfe_program p, const double* params, double* energy);
 /* Prepare U(t)|ref> into a state (build_Uvqc + apply_circuit). */
 int qfe_program_prepare(qfe_program p, const double* params, qfe_state out);
+/* qfe_program_energy through a captured hipGraph (no reference counterpart;
+ * for optimizer loops that evaluate one ansatz many times).  The first call
+ * plans the ansatz once with device-side angles and captures parameter upload,
+ * angle update, reference state, exponential passes, expectation value and
+ * result download on the library stream; every call writes the amplitudes and
+ * replays the graph.  The plan is the one the tuning in force at that first
+ * call gives.  Equals qfe_program_energy to rounding (1e-12). */
+int qfe_program_energy_graph(qfe_program p, const double* params, double* energy);
 /* Adjoint gradient dE/dt_k (measure_gradient, uccvqeabc.py:122-251): one
  * forward sweep, sigma = H psi, then a reverse sweep un-applying each
  * exponential on psi and sigma.  grad has nparam entries; energy may be NULL. */

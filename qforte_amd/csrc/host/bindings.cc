@@ -78,6 +78,13 @@ class UCCProgram {
         check(qfe_program_energy(p_, t.data(), &e), "UCCProgram.energy");
         return e;
     }
+    double energy_graph(const std::vector<double>& t) {
+        need(t);
+        double e = 0;
+        py::gil_scoped_release rel;
+        check(qfe_program_energy_graph(p_, t.data(), &e), "UCCProgram.energy_graph");
+        return e;
+    }
     std::pair<double, std::vector<double>> gradient(const std::vector<double>& t) {
         need(t);
         std::vector<double> g(nparam_);
@@ -587,6 +594,7 @@ PYBIND11_MODULE(_core, m) {
                       const std::vector<double>&, size_t, const QubitOperator&>(),
              "nqubit"_a, "ref_index"_a, "generators"_a, "param_index"_a, "scale"_a, "nparam"_a, "hamiltonian"_a)
         .def("energy", &UCCProgram::energy)
+        .def("energy_graph", &UCCProgram::energy_graph)
         .def("gradient", &UCCProgram::gradient)
         .def("prepare", &UCCProgram::prepare)
         .def("pool_gradients", &UCCProgram::pool_gradients, "amplitudes"_a, "operators"_a, "batch"_a = 64)

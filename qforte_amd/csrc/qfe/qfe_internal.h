@@ -102,6 +102,7 @@ struct HostOp {
     DevOp op;            // global-bit form
     uint64_t mix;        // mixing qubits
     uint64_t support;    // every qubit the op reads (for commutation)
+    int32_t src = -1;    // K_PEXP with a device-side angle: index into the caller's angle table
 };
 
 // Build HostOps from gates / op stream. Returns false + message on bad input.
@@ -110,6 +111,23 @@ int lower_pexp(const qfe_pauli& p, double angle, int n, std::vector<HostOp>& out
 
 // Execute a lowered op list on a state (plans fused passes and launches).
 int run_ops(qfe_state_s* s, std::vector<HostOp>& ops);
+
+// A planned op list kept for replay: the descriptor image (uploaded by the
+// caller to device memory it owns) and the launch sequence over it.  With
+// device_angles every K_PEXP lands in a fused, orbit or lane-orbit pass (never
+// a standalone pass, whose angle is a kernel argument), and `angles` lists the
+// byte offset in the image of each such op's (cos, sin) pair with the op's
+// `src` tag, so a kernel can rewrite the angles before each replay.
+struct OpPlan;
+struct AngleRef {
+    uint64_t off;
+    int32_t src;
+    uint32_t pad;
+};
+int plan_build(std::vector<HostOp>& ops, int n, bool device_angles, OpPlan** out, std::vector<char>& image,
+               std::vector<AngleRef>& angles);
+int plan_launch(const OpPlan* p, qfe_state_s* s, char* dev_image);
+void plan_free(OpPlan* p);
 
 // ---- reductions / Pauli sums ---------------------------------------------------
 struct PauliGroup {

@@ -883,7 +883,8 @@ static void grow_orbit(const std::vector<HostOp>& ops, const std::vector<char>& 
     }
 }
 
-static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::vector<Pass>& passes) {
+static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::vector<Pass>& passes,
+                 bool device_angles) {
     const int K = std::min(tu.tile_qubits, n);
     const int L = std::max(std::min(tu.low_qubits, K), K - 9);
     const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
@@ -901,6 +902,8 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
             p = std::move(orbit);
         } else if (tile.ops.size() >= 2) {
             p = std::move(tile);
+        } else if (device_angles && ops[first].op.kind == K_PEXP && n > (int)orbit.span.b.size()) {
+            p = std::move(orbit);  // a one-op orbit pass reads its angle from the descriptor image
         } else {
             // a standalone streaming pass
             p = Pass{};
@@ -1144,46 +1147,77 @@ static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count
     }
 }
 
-int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
-    if (ops.empty()) return QFE_OK;
-    Context& cx = ctx();
-    const int n = st->n;
-    merge_single_qubit_runs(ops);
-    std::vector<Pass> passes;
-    plan(ops, n, cx.tuning, passes);
+static inline size_t align_up(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
 
-    // Build all device descriptors for this call in one host buffer, upload once.
-    // Layout: [PassDesc per tile pass][DevOp arrays][U2 matrices for single passes][OrbitOp arrays]
+struct OpPlan {
+    std::vector<HostOp> ops;
+    std::vector<Pass> passes;
     std::vector<PassDesc> pdescs;
+    std::vector<size_t> pass_pd, pass_op0, single_mat, span_d;
+    std::vector<OrbitDesc> odescs;
+    size_t off_ops = 0, off_mat = 0, off_orb = 0, off_ld = 0, off_lo = 0, total = 0;
+};
+
+int plan_build(std::vector<HostOp>& ops_in, int n, bool device_angles, OpPlan** out, std::vector<char>& image,
+               std::vector<AngleRef>& angles) {
+    *out = nullptr;
+    Context& cx = ctx();
+    OpPlan* P = new OpPlan();
+    P->ops = std::move(ops_in);
+    std::vector<HostOp>& ops = P->ops;
+    merge_single_qubit_runs(ops);
+    std::vector<Pass>& passes = P->passes;
+    plan(ops, n, cx.tuning, passes, device_angles);
+
+    // Descriptor image layout: [PassDesc per tile pass][DevOp arrays][U2 matrices for single passes]
+    // [OrbitOp arrays][LaneDesc per lane-orbit pass][OrbitOp2 arrays]
     std::vector<DevOp> dops;
-    std::vector<size_t> pass_pd(passes.size(), (size_t)-1), pass_op0(passes.size(), 0);
-    std::vector<size_t> single_mat(passes.size(), (size_t)-1);
     std::vector<double2> mats;
-    std::vector<OrbitDesc> odescs(passes.size());
     std::vector<OrbitOp> oops;
     std::vector<LaneDesc> ldescs;
     std::vector<OrbitOp2> lops;
-    std::vector<size_t> span_d(passes.size(), 0);
+    // angle refs by region (0: DevOp, 1: OrbitOp, 2: OrbitOp2) and index, resolved to bytes below
+    struct Ref {
+        int region;
+        size_t index;
+        int32_t src;
+    };
+    std::vector<Ref> refs;
+    P->pass_pd.assign(passes.size(), (size_t)-1);
+    P->pass_op0.assign(passes.size(), 0);
+    P->single_mat.assign(passes.size(), (size_t)-1);
+    P->span_d.assign(passes.size(), 0);
+    P->odescs.resize(passes.size());
+    auto fail_plan = [&](int code, const char* msg) {
+        delete P;
+        return fail(code, msg);
+    };
     for (size_t pi = 0; pi < passes.size(); ++pi) {
         const Pass& p = passes[pi];
         if (p.kind == P_SINGLE) {
             const HostOp& h = ops[p.ops[0]];
+            if (device_angles && h.op.kind == K_PEXP && h.src >= 0)
+                return fail_plan(QFE_ERR_ARG, "plan: an exponential on every qubit of a 1-qubit register has no replayable pass");
             if (h.op.kind == K_U2) {
-                single_mat[pi] = mats.size();
+                P->single_mat[pi] = mats.size();
                 for (int i = 0; i < 16; ++i) mats.push_back(h.op.m[i]);
             }
             continue;
         }
         if (p.kind == P_ORBIT && p.span.b.size() > (size_t)kOrbitMax) {
-            span_d[pi] = ldescs.size();
-            pass_op0[pi] = lops.size();
+            P->span_d[pi] = ldescs.size();
+            P->pass_op0[pi] = lops.size();
             ldescs.emplace_back();
             build_orbit_lanes(p, ops, ldescs.back(), lops);
+            for (size_t j = 0; j < p.ops.size(); ++j)
+                if (ops[p.ops[j]].src >= 0) refs.push_back({2, P->pass_op0[pi] + j, ops[p.ops[j]].src});
             continue;
         }
         if (p.kind == P_ORBIT) {
-            pass_op0[pi] = oops.size();
-            build_orbit(p, ops, odescs[pi], oops);
+            P->pass_op0[pi] = oops.size();
+            build_orbit(p, ops, P->odescs[pi], oops);
+            for (size_t j = 0; j < p.ops.size(); ++j)
+                if (ops[p.ops[j]].src >= 0) refs.push_back({1, P->pass_op0[pi] + j, ops[p.ops[j]].src});
             continue;
         }
         // extend the tile with the lowest free qubits up to K bits
@@ -1197,7 +1231,7 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         while (nlow < 64 && ((b >> nlow) & 1ull)) ++nlow;
         // the planner keeps at least k - 9 low bits in every tile, so the
         // offset table of the high tile bits never exceeds 2^9 entries
-        if (pd.k - nlow > 9) return fail(QFE_ERR_STATE, "internal: fused tile has more than 9 high bits");
+        if (pd.k - nlow > 9) return fail_plan(QFE_ERR_STATE, "internal: fused tile has more than 9 high bits");
         pd.nlow = nlow;
         int local_of[64];
         for (int q = 0; q < 64; ++q) local_of[q] = -1;
@@ -1212,61 +1246,99 @@ int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
         }
         pd.nrest = (uint32_t)ri;
         pd.nops = (uint32_t)p.ops.size();
-        pass_op0[pi] = dops.size();
-        for (int oi : p.ops) dops.push_back(localize(ops[oi].op, b, local_of));
-        group_u1_runs(dops, pass_op0[pi], pd.nops, (int)pd.k);
-        pass_pd[pi] = pdescs.size();
-        pdescs.push_back(pd);
+        P->pass_op0[pi] = dops.size();
+        for (int oi : p.ops) {
+            if (ops[oi].src >= 0) refs.push_back({0, dops.size(), ops[oi].src});
+            dops.push_back(localize(ops[oi].op, b, local_of));
+        }
+        group_u1_runs(dops, P->pass_op0[pi], pd.nops, (int)pd.k);
+        P->pass_pd[pi] = P->pdescs.size();
+        P->pdescs.push_back(pd);
     }
-    const size_t bytes_pd = pdescs.size() * sizeof(PassDesc);
-    const size_t off_ops = (bytes_pd + 255) & ~(size_t)255;
+    const size_t bytes_pd = P->pdescs.size() * sizeof(PassDesc);
+    P->off_ops = align_up(bytes_pd, 256);
     const size_t bytes_ops = dops.size() * sizeof(DevOp);
-    const size_t off_mat = (off_ops + bytes_ops + 255) & ~(size_t)255;
+    P->off_mat = align_up(P->off_ops + bytes_ops, 256);
     const size_t bytes_mat = mats.size() * sizeof(double2);
-    const size_t off_orb = (off_mat + bytes_mat + 255) & ~(size_t)255;
+    P->off_orb = align_up(P->off_mat + bytes_mat, 256);
     const size_t bytes_orb = oops.size() * sizeof(OrbitOp);
-    const size_t off_ld = (off_orb + bytes_orb + 255) & ~(size_t)255;
+    P->off_ld = align_up(P->off_orb + bytes_orb, 256);
     const size_t bytes_ld = ldescs.size() * sizeof(LaneDesc);
-    const size_t off_lo = (off_ld + bytes_ld + 255) & ~(size_t)255;
+    P->off_lo = align_up(P->off_ld + bytes_ld, 256);
     const size_t bytes_lo = lops.size() * sizeof(OrbitOp2);
-    const size_t total = off_lo + bytes_lo;
-    char* dev = nullptr;
-    if (bytes_ops || bytes_mat || bytes_orb || bytes_ld) {
-        void* scr = nullptr;
-        int rc = scratch_reserve(total, &scr);
-        if (rc) return rc;
-        dev = static_cast<char*>(scr);
-        char* host = nullptr;
-        rc = staging_begin(total, reinterpret_cast<void**>(&host));
-        if (rc) return rc;
-        if (bytes_pd) std::memcpy(host, pdescs.data(), bytes_pd);
-        if (bytes_ops) std::memcpy(host + off_ops, dops.data(), bytes_ops);
-        if (bytes_mat) std::memcpy(host + off_mat, mats.data(), bytes_mat);
-        if (bytes_orb) std::memcpy(host + off_orb, oops.data(), bytes_orb);
-        if (bytes_ld) std::memcpy(host + off_ld, ldescs.data(), bytes_ld);
-        if (bytes_lo) std::memcpy(host + off_lo, lops.data(), bytes_lo);
-        rc = staging_commit(dev, total);
-        if (rc) return rc;
+    P->total = (bytes_ops || bytes_mat || bytes_orb || bytes_ld) ? P->off_lo + bytes_lo : 0;
+    image.assign(P->total, 0);
+    char* host = image.data();
+    if (bytes_pd) std::memcpy(host, P->pdescs.data(), bytes_pd);
+    if (bytes_ops) std::memcpy(host + P->off_ops, dops.data(), bytes_ops);
+    if (bytes_mat) std::memcpy(host + P->off_mat, mats.data(), bytes_mat);
+    if (bytes_orb) std::memcpy(host + P->off_orb, oops.data(), bytes_orb);
+    if (bytes_ld) std::memcpy(host + P->off_ld, ldescs.data(), bytes_ld);
+    if (bytes_lo) std::memcpy(host + P->off_lo, lops.data(), bytes_lo);
+    angles.clear();
+    for (const Ref& r : refs) {
+        uint64_t off;
+        if (r.region == 0)
+            off = P->off_ops + r.index * sizeof(DevOp) + offsetof(DevOp, m);
+        else if (r.region == 1)
+            off = P->off_orb + r.index * sizeof(OrbitOp) + offsetof(OrbitOp, c);
+        else
+            off = P->off_lo + r.index * sizeof(OrbitOp2) + offsetof(OrbitOp2, c);
+        angles.push_back(AngleRef{off, r.src, 0});
     }
-    for (size_t pi = 0; pi < passes.size(); ++pi) {
-        const Pass& p = passes[pi];
+    *out = P;
+    return QFE_OK;
+}
+
+int plan_launch(const OpPlan* P, qfe_state_s* st, char* dev) {
+    for (size_t pi = 0; pi < P->passes.size(); ++pi) {
+        const Pass& p = P->passes[pi];
         int rc;
         if (p.kind == P_SINGLE) {
-            double2* m = single_mat[pi] == (size_t)-1 ? nullptr : reinterpret_cast<double2*>(dev + off_mat) + single_mat[pi];
-            rc = launch_single(st, ops[p.ops[0]], m);
+            double2* m = P->single_mat[pi] == (size_t)-1 ? nullptr
+                                                         : reinterpret_cast<double2*>(dev + P->off_mat) + P->single_mat[pi];
+            rc = launch_single(st, P->ops[p.ops[0]], m);
         } else if (p.kind == P_ORBIT && p.span.b.size() > (size_t)kOrbitMax) {
-            rc = launch_orbit_lanes(st, (int)p.span.b.size(), reinterpret_cast<const LaneDesc*>(dev + off_ld) + span_d[pi],
-                                    reinterpret_cast<const OrbitOp2*>(dev + off_lo) + pass_op0[pi], (uint32_t)p.ops.size());
+            rc = launch_orbit_lanes(st, (int)p.span.b.size(),
+                                    reinterpret_cast<const LaneDesc*>(dev + P->off_ld) + P->span_d[pi],
+                                    reinterpret_cast<const OrbitOp2*>(dev + P->off_lo) + P->pass_op0[pi],
+                                    (uint32_t)p.ops.size());
         } else if (p.kind == P_ORBIT) {
-            const OrbitOp* oo = reinterpret_cast<const OrbitOp*>(dev + off_orb) + pass_op0[pi];
-            rc = launch_orbit(st, (int)p.span.b.size(), odescs[pi], oo, (uint32_t)p.ops.size());
+            const OrbitOp* oo = reinterpret_cast<const OrbitOp*>(dev + P->off_orb) + P->pass_op0[pi];
+            rc = launch_orbit(st, (int)p.span.b.size(), P->odescs[pi], oo, (uint32_t)p.ops.size());
         } else {
-            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + off_ops) + pass_op0[pi];
-            rc = launch_fused(st, pdescs[pass_pd[pi]], dop);
+            const DevOp* dop = reinterpret_cast<const DevOp*>(dev + P->off_ops) + P->pass_op0[pi];
+            rc = launch_fused(st, P->pdescs[P->pass_pd[pi]], dop);
         }
         if (rc) return rc;
     }
     return QFE_OK;
+}
+
+void plan_free(OpPlan* p) { delete p; }
+
+int run_ops(qfe_state_s* st, std::vector<HostOp>& ops) {
+    if (ops.empty()) return QFE_OK;
+    OpPlan* P = nullptr;
+    std::vector<char> image;
+    std::vector<AngleRef> angles;
+    int rc = plan_build(ops, st->n, false, &P, image, angles);
+    if (rc) return rc;
+    char* dev = nullptr;
+    if (P->total) {
+        void* scr = nullptr;
+        rc = scratch_reserve(P->total, &scr);
+        char* host = nullptr;
+        if (!rc) rc = staging_begin(P->total, reinterpret_cast<void**>(&host));
+        if (!rc) {
+            dev = static_cast<char*>(scr);
+            std::memcpy(host, image.data(), P->total);
+            rc = staging_commit(dev, P->total);
+        }
+    }
+    if (!rc) rc = plan_launch(P, st, dev);
+    plan_free(P);
+    return rc;
 }
 
 }  // namespace qfe

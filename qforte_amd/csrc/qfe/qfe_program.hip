@@ -69,6 +69,63 @@ static void group_many(const qfe_pauli* terms, const uint64_t* offsets, uint64_t
     }
 }
 
+// ---- captured energy sequence --------------------------------------------------------------
+// The ansatz is planned once with device-side angles: every exponential's
+// (cos, sin) lives in a descriptor image in HBM and a kernel rewrites them
+// from the amplitudes before the passes run.  Parameter upload, that kernel,
+// the reference state, the exponential passes, the expectation value and the
+// result download are captured once as a hipGraph; each energy call then
+// writes the amplitudes into pinned memory and replays the graph.
+struct DevAngle {
+    uint64_t off;  // byte offset of the (cos, sin) pair in the descriptor image
+    uint32_t param;
+    uint32_t pad;
+    double scale;  // generator scale
+    double cim;    // Im(coefficient) of the Pauli term
+};
+
+__global__ __launch_bounds__(256) void k_program_angles(const double* __restrict__ params,
+                                                        const DevAngle* __restrict__ a, uint32_t n,
+                                                        char* __restrict__ image) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevAngle d = a[i];
+    double sn, c;
+    sincos((params[d.param] * d.scale) * d.cim, &sn, &c);
+    double* dst = reinterpret_cast<double*>(image + d.off);  // 8-byte aligned in every descriptor kind
+    dst[0] = c;
+    dst[1] = sn;
+}
+
+struct ProgramGraph {
+    OpPlan* plan = nullptr;
+    char* image = nullptr;
+    DevAngle* angles = nullptr;
+    uint32_t nangles = 0;
+    double* dparams = nullptr;
+    double* hparams = nullptr;  // pinned; read by the graph's first node
+    double2* partial = nullptr;
+    double2* gout = nullptr;
+    double2* hout = nullptr;  // pinned; written by the graph's last node
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void graph_free(ProgramGraph* g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->image) (void)hipFree(g->image);
+    if (g->angles) (void)hipFree(g->angles);
+    if (g->dparams) (void)hipFree(g->dparams);
+    if (g->partial) (void)hipFree(g->partial);
+    if (g->gout) (void)hipFree(g->gout);
+    if (g->hparams) (void)hipHostFree(g->hparams);
+    if (g->hout) (void)hipHostFree(g->hout);
+    plan_free(g->plan);
+    delete g;
+}
+
 }  // namespace qfe
 
 struct qfe_program_s {
@@ -89,6 +146,7 @@ struct qfe_program_s {
     size_t gout_cap = 0;
     qfe_state psi = nullptr;
     qfe_state sig = nullptr;
+    qfe::ProgramGraph* graph = nullptr;  // captured energy sequence (first graph energy call)
 };
 
 namespace qfe {
@@ -179,6 +237,111 @@ static int energy_of(qfe_program p, qfe_state s, double* energy) {
     return QFE_OK;
 }
 
+// Plan the ansatz with device-side angles and capture the energy sequence.
+static int graph_build(qfe_program p) {
+    int rc = ensure_states(p, false);
+    if (rc) return rc;
+    // every term with a nonzero coefficient becomes an op, whatever the amplitudes
+    std::vector<HostOp> ops;
+    std::vector<DevAngle> table;
+    for (uint64_t k = 0; k + 1 < p->gen_off.size(); ++k) {
+        if (p->gscale[k] == 0.0) continue;
+        for (uint64_t i = p->gen_off[k]; i < p->gen_off[k + 1]; ++i) {
+            const qfe_pauli& t = p->gen_terms[i];
+            if (t.coeff.im == 0.0) continue;
+            rc = lower_pexp(t, 1.0, p->n, ops);
+            if (rc) return rc;
+            ops.back().src = (int32_t)table.size();
+            table.push_back(DevAngle{0, (uint32_t)p->gparam[k], 0, p->gscale[k], t.coeff.im});
+        }
+    }
+    ProgramGraph* g = new ProgramGraph();
+    std::vector<char> image;
+    std::vector<AngleRef> refs;
+    rc = plan_build(ops, p->n, true, &g->plan, image, refs);
+    if (rc) {
+        graph_free(g);
+        return rc;
+    }
+    std::vector<DevAngle> angles;
+    angles.reserve(refs.size());
+    for (const AngleRef& r : refs) {
+        DevAngle a = table[r.src];
+        a.off = r.off;
+        angles.push_back(a);
+    }
+    g->nangles = (uint32_t)angles.size();
+    const uint32_t nhg = p->ham.ngroups;
+    const uint32_t nch = expval_chunks(p->n, nhg);
+    auto alloc = [&]() -> int {
+        if (!image.empty()) QFE_HIP(hipMalloc(&g->image, image.size()));
+        if (!angles.empty()) QFE_HIP(hipMalloc(&g->angles, angles.size() * sizeof(DevAngle)));
+        QFE_HIP(hipMalloc(&g->dparams, (p->nparam + 1) * sizeof(double)));
+        QFE_HIP(hipMalloc(&g->partial, ((size_t)nhg * nch + 1) * sizeof(double2)));
+        QFE_HIP(hipMalloc(&g->gout, ((size_t)nhg + 1) * sizeof(double2)));
+        QFE_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->hparams), (p->nparam + 1) * sizeof(double), hipHostMallocDefault));
+        QFE_HIP(hipHostMalloc(reinterpret_cast<void**>(&g->hout), ((size_t)nhg + 1) * sizeof(double2), hipHostMallocDefault));
+        if (!image.empty()) QFE_HIP(hipMemcpy(g->image, image.data(), image.size(), hipMemcpyHostToDevice));
+        if (!angles.empty())
+            QFE_HIP(hipMemcpy(g->angles, angles.data(), angles.size() * sizeof(DevAngle), hipMemcpyHostToDevice));
+        return QFE_OK;
+    };
+    rc = alloc();
+    if (rc) {
+        graph_free(g);
+        return rc;
+    }
+    hipStream_t st = ctx().stream;
+    QFE_HIP(hipStreamSynchronize(st));
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        graph_free(g);
+        return hip_fail(e, "qfe_program_energy_graph: begin capture");
+    }
+    // launches below only record; the capture is always ended before returning
+    auto record = [&]() -> int {
+        if (p->nparam)
+            QFE_HIP(hipMemcpyAsync(g->dparams, g->hparams, p->nparam * sizeof(double), hipMemcpyHostToDevice, st));
+        if (g->nangles) {
+            hipLaunchKernelGGL(k_program_angles, dim3((g->nangles + 255) / 256), dim3(256), 0, st, g->dparams, g->angles,
+                               g->nangles, g->image);
+            QFE_HIP(hipGetLastError());
+        }
+        int r = launch_basis(p->psi->d, p->psi->N, p->ref, st);
+        if (!r) r = plan_launch(g->plan, p->psi, g->image);
+        if (!r) r = launch_expval_async(p->psi->d, p->psi->d, p->n, p->ham.g, nhg, p->ham.z, p->ham.c, g->partial, nch, g->gout);
+        if (r) return r;
+        if (nhg) QFE_HIP(hipMemcpyAsync(g->hout, g->gout, nhg * sizeof(double2), hipMemcpyDeviceToHost, st));
+        return QFE_OK;
+    };
+    rc = record();
+    e = hipStreamEndCapture(st, &g->graph);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "qfe_program_energy_graph: end capture");
+    if (!rc) {
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "qfe_program_energy_graph: instantiate");
+    }
+    if (rc) {
+        graph_free(g);
+        return rc;
+    }
+    p->graph = g;
+    return QFE_OK;
+}
+
+static int graph_energy(qfe_program p, const double* params, double* energy) {
+    ProgramGraph* g = p->graph;
+    hipStream_t st = ctx().stream;
+    // the previous replay has finished (every call synchronizes), so the pinned buffers are free
+    if (p->nparam) std::memcpy(g->hparams, params, p->nparam * sizeof(double));
+    QFE_HIP(hipGraphLaunch(g->exec, st));
+    QFE_HIP(hipStreamSynchronize(st));
+    double e = 0.0;
+    for (uint32_t i = 0; i < p->ham.ngroups; ++i) e += g->hout[i].x;
+    *energy = e;
+    return QFE_OK;
+}
+
 int program_nqubit(qfe_program p) { return p->n; }
 uint64_t program_nparam(qfe_program p) { return p->nparam; }
 ProgramView program_view(qfe_program p) {
@@ -253,6 +416,7 @@ int qfe_program_destroy(qfe_program p) {
     table_free(p->gens);
     if (p->partial) (void)hipFree(p->partial);
     if (p->gout) (void)hipFree(p->gout);
+    graph_free(p->graph);
     qfe_state_destroy(p->psi);
     qfe_state_destroy(p->sig);
     delete p;
@@ -272,6 +436,15 @@ int qfe_program_energy(qfe_program p, const double* params, double* energy) {
     rc = prepare(p, params, p->psi);
     if (rc) return rc;
     return energy_of(p, p->psi, energy);
+}
+
+int qfe_program_energy_graph(qfe_program p, const double* params, double* energy) {
+    if (!p || !energy || (p->nparam && !params)) return fail(QFE_ERR_ARG, "qfe_program_energy_graph: invalid argument");
+    if (!p->graph) {
+        int rc = graph_build(p);
+        if (rc) return rc;
+    }
+    return graph_energy(p, params, energy);
 }
 
 int qfe_program_gradient(qfe_program p, const double* params, double* grad, double* energy) {

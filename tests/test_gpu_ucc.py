@@ -241,3 +241,39 @@ def test_uccsd_16_qubits_against_dense_oracle(qf, oracle):
     hmap = qf.HamiltonianMap(norb)
     tpl = qf.compile_ucc_template(16, ref, hmap, pool, tops)
     assert abs(tpl.energy(t, hmap.pack(0.0, h, g)) - e_or) < E_TOL
+
+
+def test_graph_energy_replays_match_eager_energy(qf, oracle, h4):
+    """The captured energy sequence: every replay (fresh amplitudes, eager
+    program calls between replays, all-zero amplitudes whose exponentials the
+    eager path skips) equals the eager energy, and the first one the oracle."""
+    mol, pool = h4
+    n = len(mol.hf_reference)
+    tops = list(range(len(pool)))
+    prog = qf.compile_ucc_program(n, mol.hf_reference, mol.hamiltonian, pool, tops)
+    gens, _, _ = qf.ucc_generators(pool, tops)
+    rng = np.random.default_rng(12)
+    t0 = list(rng.normal(0.0, 0.1, len(tops)))
+    e_or, _ = oracle_ucc_energy(oracle, n, mol.hf_reference, gens, t0, terms_list(mol.hamiltonian))
+    assert abs(prog.energy_graph(t0) - e_or) < E_TOL
+    for k in range(6):
+        t = [0.0] * len(tops) if k == 2 else list(rng.normal(0.0, 0.1, len(tops)))
+        g = prog.energy_graph(t)
+        e = prog.energy(t)
+        assert abs(g - e) < 1e-12, (k, g, e)
+        prog.gradient(t)  # eager work on the same stream and state between replays
+    assert abs(prog.energy_graph(t0) - e_or) < E_TOL
+
+
+def test_graph_energy_with_shared_amplitudes_and_12_qubits(qf, oracle):
+    mol = qf.synthetic_molecule(6, 3, 3, seed=5)
+    for pool_type in ("SD", "sa_SD"):
+        pool = qf.SQOpPool()
+        pool.set_orb_spaces(mol.hf_reference)
+        pool.fill_pool(pool_type)
+        tops = list(range(len(pool)))
+        prog = qf.compile_ucc_program(12, mol.hf_reference, mol.hamiltonian, pool, tops, pool_type=pool_type)
+        rng = np.random.default_rng(13)
+        for _ in range(3):
+            t = list(rng.normal(0.0, 0.05, len(tops)))
+            assert abs(prog.energy_graph(t) - prog.energy(t)) < 1e-12, pool_type
