@@ -39,6 +39,54 @@ __device__ __forceinline__ uint64_t ins00(uint64_t k, uint32_t lo_b, uint32_t hi
 
 __device__ __forceinline__ double parity_sign(uint64_t v) { return (__popcll(v) & 1) ? -1.0 : 1.0; }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b mod 8).  A streaming kernel that takes data chunk xcd_block() instead of
+// blockIdx.x gives each XCD one contiguous eighth of the chunks, so each XCD
+// walks its own run of DRAM pages; on 16 GiB in-place sweeps this measured
+// 3-6 % faster than the plain mapping.  Grids that are not a multiple of 8
+// keep the plain mapping.
+__device__ __forceinline__ uint64_t xcd_block() {
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    return (n & 7u) ? (uint64_t)b : (uint64_t)(b & 7u) * (n >> 3) + (b >> 3);
+}
+
+// Streaming loads and stores that bypass cache residency (each amplitude of a
+// sweep is touched once).  They pay only when every load or store
+// instruction of a wave covers whole 128-byte lines: when the lowest bit that
+// pairs amplitudes (a gate target, an insertion or orbit pivot bit) is below
+// kNtMinBit, consecutive lanes touch every other amplitude, the partner half
+// of each line comes from another instruction, and non-temporal accesses then
+// fetch and write partial lines (measured up to 5x slower than plain
+// accesses).  States small enough to stay in the caches (below 2^24
+// amplitudes = 256 MiB, the size of the last-level cache) lose by bypassing
+// them, so the host asks for NT only from kNtMinQubits on.  Kernels take NT
+// as a template flag or argument chosen on the host (use_nt).
+constexpr uint32_t kNtMinBit = 3;
+constexpr int kNtMinQubits = 24;
+inline bool use_nt(int nqubit, uint32_t low_bit) { return nqubit >= kNtMinQubits && low_bit >= kNtMinBit; }
+
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+    return make_double2(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+template <bool NT>
+__device__ __forceinline__ double2 ld_s(const double2* p) {
+    if constexpr (NT)
+        return ld_nt(p);
+    else
+        return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_s(double2* p, double2 v) {
+    if constexpr (NT)
+        st_nt(p, v);
+    else
+        *p = v;
+}
+
 // c_i <- c_{i ^ v}: a butterfly of conditional swaps, one stage per bit of v.
 // Every array index is a compile-time constant (v only selects whether a
 // stage swaps), so the array stays in registers.

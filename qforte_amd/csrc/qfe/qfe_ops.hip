@@ -152,10 +152,11 @@ __device__ __forceinline__ void pexp_pair(double2& a, double2& b, double c, doub
 }
 
 // U1 on bit t with an optional single control bit c (cbit = 1<<c or 0).
+template <bool NT>
 __global__ __launch_bounds__(kTPB) void k_u1(double2* __restrict__ s, uint64_t nitems, uint32_t t, uint32_t c,
                                              uint64_t cbit, double2 m00, double2 m01, double2 m10, double2 m11) {
     const double2 m[4] = {m00, m01, m10, m11};
-    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint64_t base = xcd_block() * (kTPB * kUnroll) + threadIdx.x;
     const uint32_t lo = t < c ? t : c, hi = t < c ? c : t;
     double2 a[kUnroll], b[kUnroll];
     uint64_t j0[kUnroll];
@@ -164,8 +165,8 @@ __global__ __launch_bounds__(kTPB) void k_u1(double2* __restrict__ s, uint64_t n
         const uint64_t k = base + (uint64_t)u * kTPB;
         if (k < nitems) {
             j0[u] = cbit ? (ins00(k, lo, hi) | cbit) : ins0(k, t);
-            a[u] = s[j0[u]];
-            b[u] = s[j0[u] | (1ull << t)];
+            a[u] = ld_s<NT>(s + j0[u]);
+            b[u] = ld_s<NT>(s + (j0[u] | (1ull << t)));
         }
     }
 #pragma unroll
@@ -173,8 +174,8 @@ __global__ __launch_bounds__(kTPB) void k_u1(double2* __restrict__ s, uint64_t n
         const uint64_t k = base + (uint64_t)u * kTPB;
         if (k < nitems) {
             u2x2(a[u], b[u], m);
-            s[j0[u]] = a[u];
-            s[j0[u] | (1ull << t)] = b[u];
+            st_s<NT>(s + j0[u], a[u]);
+            st_s<NT>(s + (j0[u] | (1ull << t)), b[u]);
         }
     }
 }
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(kTPB) void k_u1(double2* __restrict__ s, uint64_t n
 // which: 0 = both halves, 1 = only bit=0 half, 2 = only bit=1 half.
 __global__ __launch_bounds__(kTPB) void k_diag1(double2* __restrict__ s, uint64_t nitems, uint32_t t, uint32_t c,
                                                 uint64_t cbit, double2 d0, double2 d1, int which) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint64_t base = xcd_block() * (kTPB * kUnroll) + threadIdx.x;
     const uint32_t lo = t < c ? t : c, hi = t < c ? c : t;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(kTPB) void k_diag1(double2* __restrict__ s, uint64_
 // General 4x4 on bits (t, c), basis 2*c_bit + t_bit.
 __global__ __launch_bounds__(kTPB) void k_u2(double2* __restrict__ s, uint64_t nitems, uint32_t t, uint32_t c,
                                              const double2* __restrict__ mat) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint64_t base = xcd_block() * (kTPB * kUnroll) + threadIdx.x;
     const uint32_t lo = t < c ? t : c, hi = t < c ? c : t;
     double2 m[16];
 #pragma unroll
@@ -232,9 +233,10 @@ __global__ __launch_bounds__(kTPB) void k_u2(double2* __restrict__ s, uint64_t n
 }
 
 // exp(i th P) with x != 0: pairs (ja, ja ^ x), ja with bit hb clear.
+template <bool NT>
 __global__ __launch_bounds__(kTPB) void k_pexp(double2* __restrict__ s, uint64_t nitems, uint32_t hb, uint64_t x,
                                                uint64_t z, double c, double sn, double2 ph) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint64_t base = xcd_block() * (kTPB * kUnroll) + threadIdx.x;
     double2 a[kUnroll], b[kUnroll];
     uint64_t ja[kUnroll];
 #pragma unroll
@@ -242,8 +244,8 @@ __global__ __launch_bounds__(kTPB) void k_pexp(double2* __restrict__ s, uint64_t
         const uint64_t k = base + (uint64_t)u * kTPB;
         if (k < nitems) {
             ja[u] = ins0(k, hb);
-            a[u] = s[ja[u]];
-            b[u] = s[ja[u] ^ x];
+            a[u] = ld_s<NT>(s + ja[u]);
+            b[u] = ld_s<NT>(s + (ja[u] ^ x));
         }
     }
 #pragma unroll
@@ -252,8 +254,8 @@ __global__ __launch_bounds__(kTPB) void k_pexp(double2* __restrict__ s, uint64_t
         if (k < nitems) {
             const uint64_t jb = ja[u] ^ x;
             pexp_pair(a[u], b[u], c, sn, ph, __popcll(ja[u] & z) & 1, __popcll(jb & z) & 1);
-            s[ja[u]] = a[u];
-            s[jb] = b[u];
+            st_s<NT>(s + ja[u], a[u]);
+            st_s<NT>(s + jb, b[u]);
         }
     }
 }
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(kTPB) void k_pexp(double2* __restrict__ s, uint64_t
 // exp(i th P) with x == 0: diagonal phase cos + i sin ph (-1)^{j.z}.
 __global__ __launch_bounds__(kTPB) void k_pexp_diag(double2* __restrict__ s, uint64_t N, uint64_t z, double c,
                                                     double sn, double2 ph) {
-    const uint64_t base = (uint64_t)blockIdx.x * (kTPB * kUnroll) + threadIdx.x;
+    const uint64_t base = xcd_block() * (kTPB * kUnroll) + threadIdx.x;
     const double2 fp = make_double2(c - sn * ph.y, sn * ph.x);  // cos + i sin ph
     const double2 fm = make_double2(c + sn * ph.y, -sn * ph.x); // cos - i sin ph
 #pragma unroll
@@ -286,8 +288,13 @@ static int launch_single(qfe_state_s* st, const HostOp& h, double2* dev_mat) {
     switch (o.kind) {
         case K_U1: {
             const uint64_t items = cbit ? N / 4 : N / 2;
-            hipLaunchKernelGGL(k_u1, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.t, cpos, cbit,
-                               o.m[0], o.m[1], o.m[2], o.m[3]);
+            const uint32_t low = cbit ? std::min(o.t, cpos) : o.t;
+            if (use_nt(st->n, low))
+                hipLaunchKernelGGL(k_u1<true>, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.t, cpos,
+                                   cbit, o.m[0], o.m[1], o.m[2], o.m[3]);
+            else
+                hipLaunchKernelGGL(k_u1<false>, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.t, cpos,
+                                   cbit, o.m[0], o.m[1], o.m[2], o.m[3]);
             bytes = 64.0 * (double)items;
             break;
         }
@@ -311,8 +318,12 @@ static int launch_single(qfe_state_s* st, const HostOp& h, double2* dev_mat) {
         case K_PEXP: {
             if (o.x) {
                 const uint64_t items = N / 2;
-                hipLaunchKernelGGL(k_pexp, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items, o.hb, o.x,
-                                   o.z, o.m[0].x, o.m[0].y, o.m[1]);
+                if (use_nt(st->n, o.hb))
+                    hipLaunchKernelGGL(k_pexp<true>, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items,
+                                       o.hb, o.x, o.z, o.m[0].x, o.m[0].y, o.m[1]);
+                else
+                    hipLaunchKernelGGL(k_pexp<false>, dim3(blocks_for(items)), dim3(kTPB), 0, cx.stream, st->d, items,
+                                       o.hb, o.x, o.z, o.m[0].x, o.m[0].y, o.m[1]);
                 bytes = 64.0 * (double)items;
             } else {
                 hipLaunchKernelGGL(k_pexp_diag, dim3(blocks_for(N)), dim3(kTPB), 0, cx.stream, st->d, N, o.z, o.m[0].x,
@@ -407,7 +418,7 @@ __device__ __forceinline__ void fused_group(double2* tile, uint64_t base, const 
 
 template <int KQ>
 __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <= 11 ? 4 : 1))) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
-                                                     const DevOp* __restrict__ ops, uint32_t nops) {
+                                                     uint32_t nt, const DevOp* __restrict__ ops, uint32_t nops) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
     constexpr uint32_t T = 1u << KQ;
@@ -434,8 +445,11 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
     }
     __syncthreads();
     const uint32_t lowmask = (1u << nlow) - 1u;
-    const uint64_t base = tile_base(blockIdx.x);
-    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) tile[l] = s[base | hioff[l >> nlow] | (l & lowmask)];
+    const uint64_t base = tile_base(xcd_block());
+    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
+        const double2* src = s + (base | hioff[l >> nlow] | (l & lowmask));
+        tile[l] = nt ? ld_nt(src) : *src;
+    }
     for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
         const DevOp& o = ops[oi];
@@ -557,7 +571,13 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
         }
     }
     __syncthreads();
-    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) s[base | hioff[l >> nlow] | (l & lowmask)] = tile[l];
+    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
+        double2* dst = s + (base | hioff[l >> nlow] | (l & lowmask));
+        if (nt)
+            st_nt(dst, tile[l]);
+        else
+            *dst = tile[l];
+    }
 }
 
 // Map a global mask to tile-local bit positions.
@@ -658,12 +678,12 @@ __device__ __forceinline__ void orbit_pexp(double2 (&a)[1 << M], uint32_t v, uin
 
 // The basis travels as scalar kernel arguments (a by-value struct argument
 // indexed in the unrolled loops was copied to scratch memory).
-template <int M>
+template <int M, bool NT>
 __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, uint64_t norbit, uint64_t b0,
                                                      uint64_t b1, uint64_t b2, uint64_t b3, uint32_t p0, uint32_t p1,
                                                      uint32_t p2, uint32_t p3, const OrbitOp* __restrict__ ops,
                                                      uint32_t nops) {
-    const uint64_t t = (uint64_t)blockIdx.x * kOrbitTPB + threadIdx.x;
+    const uint64_t t = xcd_block() * kOrbitTPB + threadIdx.x;
     if (t >= norbit) return;
     const uint64_t bb[4] = {b0, b1, b2, b3};
     const uint32_t pp[4] = {p0, p1, p2, p3};
@@ -681,7 +701,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
     }
     double2 a[1 << M];
 #pragma unroll
-    for (int i = 0; i < (1 << M); ++i) a[i] = s[off[i]];
+    for (int i = 0; i < (1 << M); ++i) a[i] = ld_s<NT>(s + off[i]);
     for (uint32_t o = 0; o < nops; ++o) {
         const OrbitOp op = ops[o];
         const bool pj = __popcll(j & op.z) & 1;
@@ -698,7 +718,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
         }
     }
 #pragma unroll
-    for (int i = 0; i < (1 << M); ++i) s[off[i]] = a[i];
+    for (int i = 0; i < (1 << M); ++i) st_s<NT>(s + off[i], a[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -727,7 +747,7 @@ __device__ __forceinline__ double2 shfl_xor_d2(double2 v, int mask) {
     return make_double2(__shfl_xor(v.x, mask, kWave), __shfl_xor(v.y, mask, kWave));
 }
 
-template <int Q>
+template <int Q, bool NT>
 __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__ s, uint64_t norbit,
                                                            const uint64_t* __restrict__ basis,  // 4 + Q, pivots ascending
                                                            const uint32_t* __restrict__ piv,
@@ -752,12 +772,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         off[i] = j ^ hc ^ ((i & 1) ? b0 : 0ull) ^ ((i & 2) ? b1 : 0ull) ^ ((i & 4) ? b2 : 0ull) ^ ((i & 8) ? b3 : 0ull);
-        if (live) {
-            a[i].x = __builtin_nontemporal_load(&s[off[i]].x);
-            a[i].y = __builtin_nontemporal_load(&s[off[i]].y);
-        } else {
-            a[i] = make_double2(0.0, 0.0);
-        }
+        a[i] = live ? ld_s<NT>(s + off[i]) : make_double2(0.0, 0.0);
     }
     const uint32_t hbase = h * 16u;
     for (uint32_t oi = 0; oi < nops; ++oi) {
@@ -791,10 +806,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
     }
     if (live) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            __builtin_nontemporal_store(a[i].x, &s[off[i]].x);
-            __builtin_nontemporal_store(a[i].y, &s[off[i]].y);
-        }
+        for (int i = 0; i < 16; ++i) st_s<NT>(s + off[i], a[i]);
     }
 }
 
@@ -917,21 +929,26 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
 
 template <int M>
 static void launch_orbit_k(double2* s, uint64_t norbit, const OrbitDesc& d, const OrbitOp* ops, uint32_t nops,
-                           hipStream_t st) {
+                           bool nt, hipStream_t st) {
     const unsigned blocks = (unsigned)((norbit + kOrbitTPB - 1) / kOrbitTPB);
-    hipLaunchKernelGGL(k_orbit<M>, dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, d.b[0], d.b[1], d.b[2], d.b[3],
-                       d.piv[0], d.piv[1], d.piv[2], d.piv[3], ops, nops);
+    if (nt)
+        hipLaunchKernelGGL((k_orbit<M, true>), dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, d.b[0], d.b[1], d.b[2],
+                           d.b[3], d.piv[0], d.piv[1], d.piv[2], d.piv[3], ops, nops);
+    else
+        hipLaunchKernelGGL((k_orbit<M, false>), dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, d.b[0], d.b[1], d.b[2],
+                           d.b[3], d.piv[0], d.piv[1], d.piv[2], d.piv[3], ops, nops);
 }
 
 static int launch_orbit(qfe_state_s* st, int M, const OrbitDesc& d, const OrbitOp* ops, uint32_t nops) {
     Context& cx = ctx();
     const uint64_t norbit = st->N >> M;
+    const bool nt = use_nt(st->n, M == 0 ? 63u : d.piv[0]);  // pivots ascending: piv[0] is the lowest
     switch (M) {
-        case 0: launch_orbit_k<0>(st->d, norbit, d, ops, nops, cx.stream); break;  // only diagonal exponentials
-        case 1: launch_orbit_k<1>(st->d, norbit, d, ops, nops, cx.stream); break;
-        case 2: launch_orbit_k<2>(st->d, norbit, d, ops, nops, cx.stream); break;
-        case 3: launch_orbit_k<3>(st->d, norbit, d, ops, nops, cx.stream); break;
-        case 4: launch_orbit_k<4>(st->d, norbit, d, ops, nops, cx.stream); break;
+        case 0: launch_orbit_k<0>(st->d, norbit, d, ops, nops, nt, cx.stream); break;  // only diagonal exponentials
+        case 1: launch_orbit_k<1>(st->d, norbit, d, ops, nops, nt, cx.stream); break;
+        case 2: launch_orbit_k<2>(st->d, norbit, d, ops, nops, nt, cx.stream); break;
+        case 3: launch_orbit_k<3>(st->d, norbit, d, ops, nops, nt, cx.stream); break;
+        case 4: launch_orbit_k<4>(st->d, norbit, d, ops, nops, nt, cx.stream); break;
         default: return fail(QFE_ERR_STATE, "internal: orbit dimension out of range");
     }
     QFE_HIP(hipGetLastError());
@@ -1013,17 +1030,23 @@ static void build_orbit_lanes(const Pass& p, const std::vector<HostOp>& ops, Lan
     }
 }
 
-static int launch_orbit_lanes(qfe_state_s* st, int M, const LaneDesc* dd, const OrbitOp2* ops, uint32_t nops) {
+static int launch_orbit_lanes(qfe_state_s* st, int M, uint32_t low_pivot, const LaneDesc* dd, const OrbitOp2* ops,
+                              uint32_t nops) {
     Context& cx = ctx();
     const uint64_t norbit = st->N >> M;
     const uint64_t threads = norbit << (M - kOrbitMax);
     const unsigned blocks = (unsigned)((threads + kOrbitTPB - 1) / kOrbitTPB);
     const uint64_t* b = reinterpret_cast<const uint64_t*>(dd);
     const uint32_t* piv = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(dd) + offsetof(LaneDesc, piv));
-    if (M == 5)
-        hipLaunchKernelGGL(k_orbit_lanes<1>, dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+    const bool nt = use_nt(st->n, low_pivot);
+    if (M == 5 && nt)
+        hipLaunchKernelGGL((k_orbit_lanes<1, true>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+    else if (M == 5)
+        hipLaunchKernelGGL((k_orbit_lanes<1, false>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+    else if (M == 6 && nt)
+        hipLaunchKernelGGL((k_orbit_lanes<2, true>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
     else if (M == 6)
-        hipLaunchKernelGGL(k_orbit_lanes<2>, dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+        hipLaunchKernelGGL((k_orbit_lanes<2, false>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
     else
         return fail(QFE_ERR_STATE, "internal: lane orbit dimension out of range");
     QFE_HIP(hipGetLastError());
@@ -1033,7 +1056,8 @@ static int launch_orbit_lanes(qfe_state_s* st, int M, const LaneDesc* dd, const 
 }
 
 template <int KQ>
-static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, hipStream_t st) {
+static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, const DevOp* ops, uint32_t nt,
+                           hipStream_t st) {
     const size_t lds = (size_t)sizeof(double2) << KQ;
     static bool attr_set = false;  // large tiles need the dynamic-LDS limit raised once
     if (!attr_set) {
@@ -1041,7 +1065,7 @@ static void launch_fused_k(double2* s, uint64_t nblocks, const PassDesc& pd, con
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, ops,
+    hipLaunchKernelGGL(k_fused<KQ>, dim3((unsigned)nblocks), dim3(kFusedTPB), lds, st, s, pd.bmask, pd.nlow, nt, ops,
                        pd.nops);
 }
 
@@ -1049,19 +1073,20 @@ static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) 
     Context& cx = ctx();
     const int k = (int)pd.k;
     const uint64_t nblocks = st->N >> k;
+    const uint32_t nt = use_nt(st->n, pd.nlow) ? 1u : 0u;  // tile loads and stores run over 2^nlow contiguous amplitudes
     switch (k) {
-        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, cx.stream); break;
-        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, cx.stream); break;
+        case 2: launch_fused_k<2>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 3: launch_fused_k<3>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 4: launch_fused_k<4>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 5: launch_fused_k<5>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 6: launch_fused_k<6>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 7: launch_fused_k<7>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 8: launch_fused_k<8>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 9: launch_fused_k<9>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 10: launch_fused_k<10>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 11: launch_fused_k<11>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 12: launch_fused_k<12>(st->d, nblocks, pd, dops, nt, cx.stream); break;
+        case 13: launch_fused_k<13>(st->d, nblocks, pd, dops, nt, cx.stream); break;
         default: return fail(QFE_ERR_ARG, "internal: fused tile size out of range");
     }
     QFE_HIP(hipGetLastError());
@@ -1299,7 +1324,9 @@ int plan_launch(const OpPlan* P, qfe_state_s* st, char* dev) {
                                                          : reinterpret_cast<double2*>(dev + P->off_mat) + P->single_mat[pi];
             rc = launch_single(st, P->ops[p.ops[0]], m);
         } else if (p.kind == P_ORBIT && p.span.b.size() > (size_t)kOrbitMax) {
-            rc = launch_orbit_lanes(st, (int)p.span.b.size(),
+            uint32_t low = 63;
+            for (uint64_t v : p.span.b) low = std::min(low, (uint32_t)(63 - __builtin_clzll(v)));
+            rc = launch_orbit_lanes(st, (int)p.span.b.size(), low,
                                     reinterpret_cast<const LaneDesc*>(dev + P->off_ld) + P->span_d[pi],
                                     reinterpret_cast<const OrbitOp2*>(dev + P->off_lo) + P->pass_op0[pi],
                                     (uint32_t)p.ops.size());

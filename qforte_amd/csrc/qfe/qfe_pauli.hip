@@ -229,7 +229,7 @@ struct OrbitTerm {
 
 
 // SAME: bra and ket are one state (plain expectation value): load it once.
-template <int M, bool SAME>
+template <int M, bool SAME, bool NT>
 __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restrict__ bra,
                                                          const double2* __restrict__ ket, uint64_t norbit,
                                                          uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
@@ -237,12 +237,18 @@ __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restri
                                                          const OrbitGroup* __restrict__ groups, uint32_t ngroups,
                                                          const OrbitTerm* __restrict__ terms,
                                                          double2* __restrict__ partial, uint32_t nb) {
-    __shared__ double2 acc_s[kEvOrbitGroups][kEvTPB];
+    // per-thread accumulator of each group (dynamic LDS: ngroups * kEvTPB
+    // entries, so passes with few groups keep more workgroups per CU)
+    extern __shared__ double2 acc_s[];
     __shared__ double2 lds[16];
     const uint64_t bb[4] = {b0, b1, b2, b3};
     const uint32_t pp[4] = {p0, p1, p2, p3};
-    for (uint32_t g = 0; g < ngroups; ++g) acc_s[g][threadIdx.x] = make_double2(0.0, 0.0);
-    for (uint64_t t = (uint64_t)blockIdx.x * kEvTPB + threadIdx.x; t < norbit; t += (uint64_t)gridDim.x * kEvTPB) {
+    for (uint32_t g = 0; g < ngroups; ++g) acc_s[g * kEvTPB + threadIdx.x] = make_double2(0.0, 0.0);
+    // chunk ch of nb contiguous orbit ranges (a multiple of the block size each)
+    const uint64_t ch = xcd_block();
+    const uint64_t per = ((norbit + nb - 1) / nb + kEvTPB - 1) / kEvTPB * kEvTPB;
+    const uint64_t t1 = min(norbit, (ch + 1) * per);
+    for (uint64_t t = ch * per + threadIdx.x; t < t1; t += kEvTPB) {
         uint64_t j = t;
 #pragma unroll
         for (int k = 0; k < M; ++k) j = ins0(j, pp[k]);
@@ -253,8 +259,8 @@ __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restri
 #pragma unroll
             for (int k = 0; k < M; ++k)
                 if ((i >> k) & 1) c ^= bb[k];
-            a[i] = ket[j ^ c];
-            if constexpr (!SAME) b[i] = bra[j ^ c];
+            a[i] = ld_s<NT>(ket + (j ^ c));
+            if constexpr (!SAME) b[i] = ld_s<NT>(bra + (j ^ c));
         }
         if constexpr (SAME) {
 #pragma unroll
@@ -288,12 +294,12 @@ __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restri
                 }
                 acc = cadd(acc, cmul(T.c, w));
             }
-            acc_s[g][threadIdx.x] = cadd(acc_s[g][threadIdx.x], acc);
+            acc_s[g * kEvTPB + threadIdx.x] = cadd(acc_s[g * kEvTPB + threadIdx.x], acc);
         }
     }
     for (uint32_t g = 0; g < ngroups; ++g) {
-        const double2 s = block_sum(acc_s[g][threadIdx.x], lds);
-        if (threadIdx.x == 0) partial[(uint64_t)groups[g].slot * nb + blockIdx.x] = s;
+        const double2 s = block_sum(acc_s[g * kEvTPB + threadIdx.x], lds);
+        if (threadIdx.x == 0) partial[(uint64_t)groups[g].slot * nb + ch] = s;
     }
 }
 
@@ -326,14 +332,20 @@ void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& pas
 template <int M>
 static void launch_orbit_expval_k(const double2* bra, const double2* ket, uint64_t norbit, const uint64_t* b,
                                   const uint32_t* piv, const OrbitGroup* dg, uint32_t ng, const OrbitTerm* dt,
-                                  double2* partial, uint32_t nb, hipStream_t st) {
-    if (bra == ket) {
-        hipLaunchKernelGGL((k_orbit_expval<M, true>), dim3(nb), dim3(kEvTPB), 0, st, bra, ket, norbit, b[0], b[1],
-                           b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
-    } else {
-        hipLaunchKernelGGL((k_orbit_expval<M, false>), dim3(nb), dim3(kEvTPB), 0, st, bra, ket, norbit, b[0], b[1],
-                           b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
-    }
+                                  double2* partial, uint32_t nb, bool nt, hipStream_t st) {
+    const size_t lds = (size_t)ng * kEvTPB * sizeof(double2);
+    if (bra == ket && nt)
+        hipLaunchKernelGGL((k_orbit_expval<M, true, true>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
+                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    else if (bra == ket)
+        hipLaunchKernelGGL((k_orbit_expval<M, true, false>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
+                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    else if (nt)
+        hipLaunchKernelGGL((k_orbit_expval<M, false, true>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
+                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    else
+        hipLaunchKernelGGL((k_orbit_expval<M, false, false>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
+                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
 }
 
 static int expval_groups_orbit(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
@@ -419,12 +431,13 @@ static int expval_groups_orbit(const double2* bra, const double2* ket, int n, co
         const uint64_t norbit = N >> L.M;
         const OrbitGroup* g = dg + L.g0;
         const uint32_t ng = (uint32_t)L.ng;
+        const bool nt = use_nt(n, L.M == 0 ? 63u : L.piv[0]);  // pivots ascending
         switch (L.M) {
-            case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
-            case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
-            case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
-            case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
-            case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, cx.stream); break;
+            case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+            case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+            case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+            case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+            case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
             default: return fail(QFE_ERR_STATE, "internal: expectation orbit dimension out of range");
         }
         QFE_HIP(hipGetLastError());
