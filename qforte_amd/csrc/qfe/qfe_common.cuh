@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace qfe {
 
@@ -103,6 +104,23 @@ __device__ __forceinline__ void xor_permute(double2 (&c)[1 << M], uint32_t v) {
                 c[i | (1 << k)] = t;
             }
         }
+    }
+}
+
+// Call f(std::integral_constant<uint32_t, V>{}) for the V in [0, 2^M) equal to
+// the uniform runtime value v: a chain of scalar compares, after which every
+// array index in f that depends on V is a compile-time constant (a partner
+// a[i ^ V] is a register name, not a data movement).
+template <int M, uint32_t V = 0, typename F>
+__device__ __forceinline__ void dispatch_mask(uint32_t v, F&& f) {
+    if constexpr (V + 1 < (1u << M)) {
+        if (v == V) {
+            f(std::integral_constant<uint32_t, V>{});
+            return;
+        }
+        dispatch_mask<M, V + 1>(v, static_cast<F&&>(f));
+    } else {
+        f(std::integral_constant<uint32_t, V>{});
     }
 }
 

@@ -663,17 +663,25 @@ struct OrbitDesc {
 template <int M>
 __device__ __forceinline__ void orbit_pexp(double2 (&a)[1 << M], uint32_t v, uint32_t sm, double c, double sn,
                                            double2 ph) {
-    double2 p[1 << M];
-#pragma unroll
-    for (int i = 0; i < (1 << M); ++i) p[i] = a[i];
-    xor_permute<M>(p, v);
     const double2 iph = make_double2(-ph.y, ph.x);  // i * phase
+    dispatch_mask<M>(v, [&](auto vc) {
+        constexpr int V = (int)decltype(vc)::value;
+        // pair by pair (i, k = i ^ V), both updated from the old values
 #pragma unroll
-    for (int i = 0; i < (1 << M); ++i) {
-        const double s = ((sm >> ((uint32_t)i ^ v)) & 1u) ? -sn : sn;
-        const double2 t = cmul(iph, p[i]);
-        a[i] = make_double2(c * a[i].x + s * t.x, c * a[i].y + s * t.y);
-    }
+        for (int i = 0; i < (1 << M); ++i) {
+            const int k = i ^ V;
+            if (k < i) continue;
+            const double2 x = a[i], y = a[k];
+            const double si = ((sm >> k) & 1u) ? -sn : sn;  // the sign belongs to the partner
+            const double2 ty = cmul(iph, y);
+            a[i] = make_double2(c * x.x + si * ty.x, c * x.y + si * ty.y);
+            if (k != i) {
+                const double sk = ((sm >> i) & 1u) ? -sn : sn;
+                const double2 tx = cmul(iph, x);
+                a[k] = make_double2(c * y.x + sk * tx.x, c * y.y + sk * tx.y);
+            }
+        }
+    });
 }
 
 // The basis travels as scalar kernel arguments (a by-value struct argument
@@ -786,23 +794,30 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
             for (int i = 0; i < 16; ++i) a[i] = cmul(((sm >> (hbase + i)) & 1ull) ? fm : fp, a[i]);
             continue;
         }
-        double2 p[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) p[i] = a[i];
-        xor_permute<4>(p, op.v & 15u);
+        // the partner of element i = h * 16 + i' is (h ^ vq) * 16 + (i' ^ V), V = v & 15, vq = v >> 4; its sign
+        // bit in sm is bit i' ^ V of the 16-bit word of group h ^ vq
         const uint32_t vq = op.v >> 4;
-        if (vq) {
-            const int lm = (int)(vq * OPW);
+        const uint32_t w = (uint32_t)(sm >> (hbase ^ (vq << 4))) & 0xFFFFu;
+        const int lm = (int)(vq * OPW);
+        const double c = op.c, s0 = op.sn;
+        dispatch_mask<4>(op.v & 15u, [&](auto vc) {
+            constexpr int V = (int)decltype(vc)::value;
+            // partner of element i: element i ^ V of this lane (lm == 0) or of lane ^ lm
+            double2 p[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) p[i] = shfl_xor_d2(p[i], lm);
-        }
+            for (int i = 0; i < 16; ++i) p[i] = a[i ^ V];
+            if (lm) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            // a_i' = cos a_i + i sin ph (-1)^{s(i ^ v)} a_{i ^ v}, i = h * 16 + i'
-            const double sn = ((sm >> ((hbase + i) ^ op.v)) & 1ull) ? -op.sn : op.sn;
-            const double2 t = cmul(iph, p[i]);
-            a[i] = make_double2(op.c * a[i].x + sn * t.x, op.c * a[i].y + sn * t.y);
-        }
+                for (int i = 0; i < 16; ++i) p[i] = shfl_xor_d2(p[i], lm);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                // a_i' = cos a_i + i sin ph (-1)^{s(partner)} a_partner
+                const double sn = ((w >> (i ^ V)) & 1u) ? -s0 : s0;
+                const double2 t = cmul(iph, p[i]);
+                a[i] = make_double2(c * a[i].x + sn * t.x, c * a[i].y + sn * t.y);
+            }
+        });
     }
     if (live) {
 #pragma unroll

@@ -223,14 +223,14 @@ struct OrbitGroup {
 struct OrbitTerm {
     uint64_t z;
     uint32_t smask;  // bit i = parity(c_i & z)
-    uint32_t pad;
+    uint32_t pmask;  // bit i = bit i ^ v of smask (v: the group's X mask in orbit coordinates)
     double2 c;       // coefficient * phase
 };
 
 
 // SAME: bra and ket are one state (plain expectation value): load it once.
 template <int M, bool SAME, bool NT>
-__global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restrict__ bra,
+__global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(SAME ? 3 : 2))) void k_orbit_expval(const double2* __restrict__ bra,
                                                          const double2* __restrict__ ket, uint64_t norbit,
                                                          uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
                                                          uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
@@ -268,6 +268,7 @@ __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restri
         }
         for (uint32_t g = 0; g < ngroups; ++g) {
             const OrbitGroup G = groups[g];
+            double2 acc = make_double2(0.0, 0.0);
             // p_i = conj(bra_i) ket_{i ^ v}
             double2 p[1 << M];
 #pragma unroll
@@ -275,16 +276,15 @@ __global__ __launch_bounds__(kEvTPB) void k_orbit_expval(const double2* __restri
             xor_permute<M>(p, G.v);
 #pragma unroll
             for (int i = 0; i < (1 << M); ++i) p[i] = cconjmul(b[i], p[i]);
-            double2 acc = make_double2(0.0, 0.0);
             for (uint32_t tt = 0; tt < G.count; ++tt) {
                 const OrbitTerm T = terms[G.first + tt];
                 const uint32_t flip = (__popcll(j & T.z) & 1) ? ~0u : 0u;
-                const uint32_t sm = T.smask ^ flip;
-                // w = sum_i (-1)^{s(i ^ v)} p_i: the sign belongs to the ket element i ^ v
+                // bit i of pmask: the sign of the ket element i ^ v
+                const uint32_t sm = T.pmask ^ flip;
                 double2 w = make_double2(0.0, 0.0);
 #pragma unroll
                 for (int i = 0; i < (1 << M); ++i) {
-                    if ((sm >> ((uint32_t)i ^ G.v)) & 1u) {
+                    if ((sm >> i) & 1u) {
                         w.x -= p[i].x;
                         w.y -= p[i].y;
                     } else {
@@ -402,6 +402,8 @@ static int expval_groups_orbit(const double2* bra, const double2* ket, int n, co
                 T.z = zs[pg.first + t];
                 for (int i = 0; i < (1 << L.M); ++i)
                     if (__builtin_popcountll(coff[i] & T.z) & 1) T.smask |= 1u << i;
+                for (int i = 0; i < (1 << L.M); ++i)
+                    if ((T.smask >> (i ^ o.v)) & 1u) T.pmask |= 1u << i;
                 T.c = cs[pg.first + t];
                 ot.push_back(T);
             }
