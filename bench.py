@@ -33,10 +33,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per k_orbit_lanes<2> launch at 30 qubits from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388672 and 16777227), each
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388756 and 16786200), each
 # scaled by the same counter's reading on k_scale, a plain sweep that reads
-# and writes exactly 16 GiB (factors 2.0000 and 1.0000): 33554571 KiB.
-TRAFFIC_PER_LAUNCH = {30: 33554571 * 1024}
+# and writes exactly 16 GiB (factors 2.0000 and 1.0000): 33563712 KiB.
+TRAFFIC_PER_LAUNCH = {30: 33563712 * 1024}
 METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
 
 
