@@ -32,7 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per k_orbit_lanes<2> launch at 30 qubits from rocprofv3 --pmc
+# HBM bytes per k_orbit_lanes<4, 2> launch at 30 qubits from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388756 and 16786200), each
 # scaled by the same counter's reading on k_scale, a plain sweep that reads
 # and writes exactly 16 GiB (factors 2.0000 and 1.0000): 33563712 KiB.
@@ -504,7 +504,7 @@ def main():
         achieved = 2 * S / (avg_ms * 1e-3) / 1e9
         single_ms = time_single_sweep(qf, W, qc, n)
         roofline = {
-            "kernel": ("k_orbit_lanes<2> (6-dimension register orbits over 4 lanes; the exponential stage: %d passes "
+            "kernel": ("k_orbit_lanes<4, 2> (6-dimension register orbits over 4 lanes of 16 amplitudes; the exponential stage: %d passes "
                        "for %d exponentials, 2S = %.2f GB per launch)" % (passes, len(strings), 2 * S / 1e9)),
             "bound": "hbm",
             "achieved": round(achieved, 1),

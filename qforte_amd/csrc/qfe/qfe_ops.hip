@@ -755,13 +755,15 @@ __device__ __forceinline__ double2 shfl_xor_d2(double2 v, int mask) {
     return make_double2(__shfl_xor(v.x, mask, kWave), __shfl_xor(v.y, mask, kWave));
 }
 
-template <int Q, bool NT>
+template <int R, int Q, bool NT>
 __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__ s, uint64_t norbit,
-                                                           const uint64_t* __restrict__ basis,  // 4 + Q, pivots ascending
+                                                           const uint64_t* __restrict__ basis,  // R + Q, pivots ascending
                                                            const uint32_t* __restrict__ piv,
                                                            const OrbitOp2* __restrict__ ops, uint32_t nops) {
-    constexpr int M = 4 + Q;
+    constexpr int M = R + Q;
+    constexpr int E = 1 << R;              // amplitudes per lane
     constexpr uint32_t OPW = kWave >> Q;  // orbits per wave
+    constexpr uint32_t WMASK = (E == 32) ? 0xFFFFFFFFu : ((1u << E) - 1u);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = ((uint64_t)blockIdx.x * kOrbitTPB + threadIdx.x) / kWave;
     const uint64_t o = wave * OPW + (lane & (OPW - 1));
@@ -773,16 +775,22 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
     for (int k = 0; k < M; ++k) j = ins0(j, piv[k]);
     uint64_t hc = 0;
     for (int k = 0; k < Q; ++k)
-        if ((h >> k) & 1u) hc ^= basis[4 + k];
-    const uint64_t b0 = basis[0], b1 = basis[1], b2 = basis[2], b3 = basis[3];
-    uint64_t off[16];
-    double2 a[16];
+        if ((h >> k) & 1u) hc ^= basis[R + k];
+    uint64_t bl[R];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        off[i] = j ^ hc ^ ((i & 1) ? b0 : 0ull) ^ ((i & 2) ? b1 : 0ull) ^ ((i & 4) ? b2 : 0ull) ^ ((i & 8) ? b3 : 0ull);
+    for (int k = 0; k < R; ++k) bl[k] = basis[k];
+    uint64_t off[E];
+    double2 a[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        uint64_t x = j ^ hc;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if ((i >> k) & 1) x ^= bl[k];
+        off[i] = x;
         a[i] = live ? ld_s<NT>(s + off[i]) : make_double2(0.0, 0.0);
     }
-    const uint32_t hbase = h * 16u;
+    const uint32_t hbase = h << R;
     for (uint32_t oi = 0; oi < nops; ++oi) {
         const OrbitOp2 op = ops[oi];
         const uint64_t sm = op.smask ^ ((__popcll(j & op.z) & 1) ? ~0ull : 0ull);
@@ -791,27 +799,27 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
             const double2 fp = make_double2(op.c - op.sn * op.ph.y, op.sn * op.ph.x);
             const double2 fm = make_double2(op.c + op.sn * op.ph.y, -op.sn * op.ph.x);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) a[i] = cmul(((sm >> (hbase + i)) & 1ull) ? fm : fp, a[i]);
+            for (int i = 0; i < E; ++i) a[i] = cmul(((sm >> (hbase + i)) & 1ull) ? fm : fp, a[i]);
             continue;
         }
-        // the partner of element i = h * 16 + i' is (h ^ vq) * 16 + (i' ^ V), V = v & 15, vq = v >> 4; its sign
-        // bit in sm is bit i' ^ V of the 16-bit word of group h ^ vq
-        const uint32_t vq = op.v >> 4;
-        const uint32_t w = (uint32_t)(sm >> (hbase ^ (vq << 4))) & 0xFFFFu;
+        // the partner of element i = h * E + i' is (h ^ vq) * E + (i' ^ V), V = v mod E, vq = v / E; its
+        // sign bit in sm is bit i' ^ V of the E-bit word of group h ^ vq
+        const uint32_t vq = op.v >> R;
+        const uint32_t w = (uint32_t)(sm >> (hbase ^ (vq << R))) & WMASK;
         const int lm = (int)(vq * OPW);
         const double c = op.c, s0 = op.sn;
-        dispatch_mask<4>(op.v & 15u, [&](auto vc) {
+        dispatch_mask<R>(op.v & (uint32_t)(E - 1), [&](auto vc) {
             constexpr int V = (int)decltype(vc)::value;
             // partner of element i: element i ^ V of this lane (lm == 0) or of lane ^ lm
-            double2 p[16];
+            double2 p[E];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) p[i] = a[i ^ V];
+            for (int i = 0; i < E; ++i) p[i] = a[i ^ V];
             if (lm) {
 #pragma unroll
-                for (int i = 0; i < 16; ++i) p[i] = shfl_xor_d2(p[i], lm);
+                for (int i = 0; i < E; ++i) p[i] = shfl_xor_d2(p[i], lm);
             }
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
+            for (int i = 0; i < E; ++i) {
                 // a_i' = cos a_i + i sin ph (-1)^{s(partner)} a_partner
                 const double sn = ((w >> (i ^ V)) & 1u) ? -s0 : s0;
                 const double2 t = cmul(iph, p[i]);
@@ -821,7 +829,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
     }
     if (live) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) st_s<NT>(s + off[i], a[i]);
+        for (int i = 0; i < E; ++i) st_s<NT>(s + off[i], a[i]);
     }
 }
 
@@ -1045,23 +1053,35 @@ static void build_orbit_lanes(const Pass& p, const std::vector<HostOp>& ops, Lan
     }
 }
 
+// In-lane dimensions of a lane orbit: 2^kLaneBits amplitudes per lane, the
+// other M - kLaneBits dimensions spread over lanes.
+constexpr int kLaneBits = 4;
+
+template <int M, bool NT>
+static void launch_orbit_lanes_k(double2* s, uint64_t norbit, const uint64_t* b, const uint32_t* piv,
+                                 const OrbitOp2* ops, uint32_t nops, hipStream_t st) {
+    constexpr int Q = M - kLaneBits;
+    const uint64_t threads = norbit << Q;
+    const unsigned blocks = (unsigned)((threads + kOrbitTPB - 1) / kOrbitTPB);
+    hipLaunchKernelGGL((k_orbit_lanes<kLaneBits, Q, NT>), dim3(blocks), dim3(kOrbitTPB), 0, st, s, norbit, b, piv, ops,
+                       nops);
+}
+
 static int launch_orbit_lanes(qfe_state_s* st, int M, uint32_t low_pivot, const LaneDesc* dd, const OrbitOp2* ops,
                               uint32_t nops) {
     Context& cx = ctx();
     const uint64_t norbit = st->N >> M;
-    const uint64_t threads = norbit << (M - kOrbitMax);
-    const unsigned blocks = (unsigned)((threads + kOrbitTPB - 1) / kOrbitTPB);
     const uint64_t* b = reinterpret_cast<const uint64_t*>(dd);
     const uint32_t* piv = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(dd) + offsetof(LaneDesc, piv));
     const bool nt = use_nt(st->n, low_pivot);
     if (M == 5 && nt)
-        hipLaunchKernelGGL((k_orbit_lanes<1, true>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+        launch_orbit_lanes_k<5, true>(st->d, norbit, b, piv, ops, nops, cx.stream);
     else if (M == 5)
-        hipLaunchKernelGGL((k_orbit_lanes<1, false>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+        launch_orbit_lanes_k<5, false>(st->d, norbit, b, piv, ops, nops, cx.stream);
     else if (M == 6 && nt)
-        hipLaunchKernelGGL((k_orbit_lanes<2, true>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+        launch_orbit_lanes_k<6, true>(st->d, norbit, b, piv, ops, nops, cx.stream);
     else if (M == 6)
-        hipLaunchKernelGGL((k_orbit_lanes<2, false>), dim3(blocks), dim3(kOrbitTPB), 0, cx.stream, st->d, norbit, b, piv, ops, nops);
+        launch_orbit_lanes_k<6, false>(st->d, norbit, b, piv, ops, nops, cx.stream);
     else
         return fail(QFE_ERR_STATE, "internal: lane orbit dimension out of range");
     QFE_HIP(hipGetLastError());
