@@ -174,6 +174,19 @@ __global__ __launch_bounds__(256) void k_sec_prologue(const SlotDesc* __restrict
     }
 }
 
+// One pair (r1, r2) of an entry with old values (a, b), j1 the determinant
+// of r1: psi_1' = c a + s s(j2) uhi b, psi_2' = c b + s s(j1) ulo a.  Every
+// kernel that rotates pairs calls this, so a pair's result does not depend on
+// which kernel or launch shape applied it.
+__device__ __forceinline__ void pair_rotate(double2* __restrict__ psi, uint64_t r1, uint64_t r2, double2 a, double2 b,
+                                            uint64_t j1, uint64_t x, uint64_t z0, double c, double s, double2 ulo,
+                                            double2 uhi) {
+    const double s1 = s * parity_sign(j1 & z0), s2 = s * parity_sign((j1 ^ x) & z0);
+    const double2 hb = cmul(uhi, b), la = cmul(ulo, a);
+    psi[r1] = make_double2(c * a.x + s2 * hb.x, c * a.y + s2 * hb.y);
+    psi[r2] = make_double2(c * b.x + s1 * la.x, c * b.y + s1 * la.y);
+}
+
 // Rotate every pair of entry `en` by (c, sgn * s); pairs of one step are
 // disjoint, so kUnroll of them are in flight per thread (loads first).
 template <int THREADS>
@@ -210,13 +223,7 @@ __device__ __forceinline__ void rotate_entry(const SecTables* __restrict__ T, co
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-            if (i0 + u * THREADS < total) {
-                const double s1 = s * parity_sign(j1[u] & z0), s2 = s * parity_sign((j1[u] ^ x) & z0);
-                // psi_1' = c a + s s(j2) uhi b ; psi_2' = c b + s s(j1) ulo a
-                const double2 hb = cmul(uhi, b[u]), la = cmul(ulo, a[u]);
-                psi[r1[u]] = make_double2(c * a[u].x + s2 * hb.x, c * a[u].y + s2 * hb.y);
-                psi[r2[u]] = make_double2(c * b[u].x + s1 * la.x, c * b[u].y + s1 * la.y);
-            }
+            if (i0 + u * THREADS < total) pair_rotate(psi, r1[u], r2[u], a[u], b[u], j1[u], x, z0, c, s, ulo, uhi);
         }
     }
 }
@@ -464,6 +471,141 @@ __global__ __launch_bounds__(256) void k_sec_pool(const SecTables* __restrict__ 
     if (threadIdx.x == 0) out[blockIdx.x] = 2.0 * tot.x;
 }
 
+// ---- grid-wide single-request kernels (large sectors) ------------------------------------
+// A lone request of a large sector would run its ansatz and reverse sweep in
+// one workgroup.  These kernels spread one step (or one generator) over the
+// whole grid instead; the host launches one per step, in order, so kernel
+// boundaries order the steps.  Pair t of a step is taken in entry order.
+constexpr int kWideTPB = 256;
+
+__device__ __forceinline__ bool step_pair(const SecTables* __restrict__ T, uint32_t e, uint32_t e_end, uint32_t t,
+                                          uint32_t& entry, uint32_t& idx) {
+    for (; e < e_end; ++e) {
+        const uint32_t cnt = T->entries[e].la_len * T->entries[e].lb_len;
+        if (t < cnt) {
+            entry = e;
+            idx = t;
+            return true;
+        }
+        t -= cnt;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void pair_of(const SecTables* __restrict__ T, const SecEntry& en, uint32_t i, uint64_t& r1,
+                                        uint64_t& r2, uint64_t& j1) {
+    uint32_t ia, ib;
+    divmod_u(i, en.lb_len, en.inv_lb, ia, ib);
+    const int2 pa = T->lists[en.la_off + ia];
+    const int2 pb = T->lists[en.lb_off + ib];
+    r1 = (uint64_t)pa.x * T->Db + (uint64_t)pb.x;
+    r2 = (uint64_t)pa.y * T->Db + (uint64_t)pb.y;
+    j1 = T->afull[pa.x] | T->bfull[pb.x];
+}
+
+// Rotate the pairs of step `step` by (c, s) (inverse: (c, -s)); `both` also rotates sigma.
+__global__ __launch_bounds__(kWideTPB) void k_sec_step(const SlotDesc* __restrict__ slots, uint32_t step, int inverse,
+                                                       int both) {
+    const SlotDesc sd = slots[0];
+    const SecTables* __restrict__ T = sd.t;
+    const SecStep st = T->steps[step];
+    uint32_t e, i;
+    if (!step_pair(T, st.e_first, st.e_first + st.e_count, blockIdx.x * kWideTPB + threadIdx.x, e, i)) return;
+    const SecEntry& en = T->entries[e];
+    const double2 cs = sd.cs[e];
+    const double sn = inverse ? -cs.y : cs.y;
+    uint64_t r1, r2, j1;
+    pair_of(T, en, i, r1, r2, j1);
+    pair_rotate(sd.psi, r1, r2, sd.psi[r1], sd.psi[r2], j1, en.x, en.z0, cs.x, sn, en.ulo, en.uhi);
+    if (both) pair_rotate(sd.sig, r1, r2, sd.sig[r1], sd.sig[r2], j1, en.x, en.z0, cs.x, sn, en.ulo, en.uhi);
+}
+
+// <sigma|K|psi> contribution of one pair (the entry_overlap formula).
+__device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, double2 pa, double2 pb, double2 sa,
+                                               double2 sb) {
+    const double s1 = en.w * parity_sign(j1 & en.z0), s2 = en.w * parity_sign((j1 ^ en.x) & en.z0);
+    const double2 k1 = cmul(en.uhi, pb), k2 = cmul(en.ulo, pa);
+    return s2 * (sa.x * k1.x + sa.y * k1.y) + s1 * (sb.x * k2.x + sb.y * k2.y);
+}
+
+// Generator with one step: its overlap and its un-rotation of psi and sigma
+// read the same four old values of each pair, so one pass does both.
+// partial[blockIdx.x] receives the block's overlap sum.
+__global__ __launch_bounds__(kWideTPB) void k_sec_genrev(const SlotDesc* __restrict__ slots, uint32_t step,
+                                                         double* __restrict__ partial) {
+    __shared__ double2 red[16];
+    const SlotDesc sd = slots[0];
+    const SecTables* __restrict__ T = sd.t;
+    const SecStep st = T->steps[step];
+    double acc = 0.0;
+    uint32_t e, i;
+    if (step_pair(T, st.e_first, st.e_first + st.e_count, blockIdx.x * kWideTPB + threadIdx.x, e, i)) {
+        const SecEntry& en = T->entries[e];
+        const double2 cs = sd.cs[e];
+        uint64_t r1, r2, j1;
+        pair_of(T, en, i, r1, r2, j1);
+        const double2 pa = sd.psi[r1], pb = sd.psi[r2], sa = sd.sig[r1], sb = sd.sig[r2];
+        acc = pair_overlap(en, j1, pa, pb, sa, sb);
+        pair_rotate(sd.psi, r1, r2, pa, pb, j1, en.x, en.z0, cs.x, -cs.y, en.ulo, en.uhi);
+        pair_rotate(sd.sig, r1, r2, sa, sb, j1, en.x, en.z0, cs.x, -cs.y, en.ulo, en.uhi);
+    }
+    const double2 tot = block_sum(make_double2(acc, 0.0), red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot.x;
+}
+
+// Generator with several steps: the overlap over all of its entries first
+// (steps [s0, s1), their entries contiguous), un-rotation by k_sec_step after.
+__global__ __launch_bounds__(kWideTPB) void k_sec_genov(const SlotDesc* __restrict__ slots, uint32_t s0, uint32_t s1,
+                                                        double* __restrict__ partial) {
+    __shared__ double2 red[16];
+    const SlotDesc sd = slots[0];
+    const SecTables* __restrict__ T = sd.t;
+    const uint32_t e0 = T->steps[s0].e_first, e1 = T->steps[s1 - 1].e_first + T->steps[s1 - 1].e_count;
+    double acc = 0.0;
+    uint32_t e, i;
+    if (step_pair(T, e0, e1, blockIdx.x * kWideTPB + threadIdx.x, e, i)) {
+        const SecEntry& en = T->entries[e];
+        uint64_t r1, r2, j1;
+        pair_of(T, en, i, r1, r2, j1);
+        acc = pair_overlap(en, j1, sd.psi[r1], sd.psi[r2], sd.sig[r1], sd.sig[r2]);
+    }
+    const double2 tot = block_sum(make_double2(acc, 0.0), red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot.x;
+}
+
+// Re<psi|sigma> block partials (the energy of the gradient path).
+__global__ __launch_bounds__(kWideTPB) void k_sec_edot(const SlotDesc* __restrict__ slots, double* __restrict__ partial) {
+    __shared__ double2 red[16];
+    const SlotDesc sd = slots[0];
+    const uint64_t D = sd.t->D;
+    const uint64_t r = (uint64_t)blockIdx.x * kWideTPB + threadIdx.x;
+    double e = 0.0;
+    if (r < D) e = sd.psi[r].x * sd.sig[r].x + sd.psi[r].y * sd.sig[r].y;
+    const double2 tot = block_sum(make_double2(e, 0.0), red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot.x;
+}
+
+// Fixed-order sums: out = sum of partial[seg[0]], gacc[k] = sum of partial[seg[k + 1]].
+__global__ __launch_bounds__(kWideTPB) void k_sec_wsum(const SlotDesc* __restrict__ slots, const double* __restrict__ partial,
+                                                       const uint2* __restrict__ seg, uint32_t nseg) {
+    const uint32_t k = blockIdx.x * kWideTPB + threadIdx.x;
+    if (k >= nseg) return;
+    const uint2 sg = seg[k];
+    double acc = 0.0;
+    for (uint32_t q = 0; q < sg.y; ++q) acc += partial[sg.x + q];
+    if (k == 0)
+        *slots[0].out = acc;
+    else
+        slots[0].gacc[k - 1] = acc;
+}
+
+// The reference determinant (grid over the sector).
+__global__ __launch_bounds__(kWideTPB) void k_sec_ref(const SlotDesc* __restrict__ slots) {
+    const SlotDesc sd = slots[0];
+    const uint64_t r = (uint64_t)blockIdx.x * kWideTPB + threadIdx.x;
+    if (r < sd.t->D) sd.psi[r] = make_double2(r == sd.t->ref_rank ? 1.0 : 0.0, 0.0);
+}
+
 // Scatter a sector state into a zeroed full state.
 __global__ void k_sector_scatter(const double2* __restrict__ psi, uint64_t D, uint32_t Db, const uint64_t* __restrict__ afull,
                                  const uint64_t* __restrict__ bfull, double2* __restrict__ out) {
@@ -693,6 +835,12 @@ struct qfe_template_s {
     size_t off_psi = 0, off_coef = 0, off_cs = 0, off_part = 0, off_sig = 0, off_gacc = 0;
     size_t work_bytes = 0, grad_bytes = 0, io_count = 0, request_bytes = 0, grad_request_bytes = 0;
     void* arena = nullptr;  // single-request arena: [desc][out][io][prefix][work incl. gradient parts]
+    // grid-wide single-request path (large sectors): host step structure and
+    // device [segments (uint2 x (ngen + 1))][block partials]
+    std::vector<uint32_t> h_gen_step, step_pairs;
+    std::vector<uint2> wide_seg;  // (first partial, count): the energy, then generator k
+    uint32_t wide_parts = 0;
+    void* wide = nullptr;
 };
 
 namespace qfe {
@@ -835,6 +983,11 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
             g = g1;
         }
     }
+    t->h_gen_step = gen_step;
+    t->step_pairs.assign(steps.size(), 0);
+    for (size_t k = 0; k < steps.size(); ++k)
+        for (uint32_t e = steps[k].e_first; e < steps[k].e_first + steps[k].e_count; ++e)
+            t->step_pairs[k] += entries[e].la_len * entries[e].lb_len;
     t->has_map = map_off != nullptr;
     t->nterms = (uint32_t)tz.size();
     t->nentries = (uint32_t)entries.size();
@@ -940,6 +1093,21 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     t->grad_bytes = o;
     t->io_count = nparam + ninputs;
     t->request_bytes = t->work_bytes + align256(t->io_count * 8 + 8) + sizeof(SlotDesc) + 16;
+    if (t->tier == 2) {  // partial-sum segments of the grid-wide reverse sweep
+        auto blocks = [](uint64_t items) { return (uint32_t)((items + kWideTPB - 1) / kWideTPB); };
+        t->wide_seg.push_back(make_uint2(0, blocks(t->D)));
+        uint32_t at = blocks(t->D);
+        for (uint64_t k = 0; k < ngen; ++k) {
+            uint64_t pairs = 0;
+            for (uint32_t sdx = gen_step[k]; sdx < gen_step[k + 1]; ++sdx) pairs += t->step_pairs[sdx];
+            t->wide_seg.push_back(make_uint2(at, blocks(pairs)));
+            at += blocks(pairs);
+        }
+        t->wide_parts = at;
+        const size_t seg_bytes = align256(t->wide_seg.size() * sizeof(uint2));
+        QFE_HIP(hipMalloc(&t->wide, seg_bytes + (size_t)t->wide_parts * 8 + 8));
+        QFE_HIP(hipMemcpy(t->wide, t->wide_seg.data(), t->wide_seg.size() * sizeof(uint2), hipMemcpyHostToDevice));
+    }
     t->grad_request_bytes = t->grad_bytes + align256(t->io_count * 8 + 8) + sizeof(SlotDesc) + 16 + ngen * 8;
     return QFE_OK;
 }
@@ -989,6 +1157,80 @@ int launch_gradient(const std::vector<qfe_template>& ts, int tier, size_t lds, c
     hipLaunchKernelGGL(k_sec_hpsi, dim3((unsigned)chunks, k), dim3(256), 0, st, ddesc);
     QFE_HIP(hipGetLastError());
     return launch_reverse(tier, k, st, ddesc);
+}
+
+// ---- grid-wide single-request sequences (tier 2) -----------------------------------------
+uint32_t wide_blocks(uint64_t items) { return (uint32_t)((items + kWideTPB - 1) / kWideTPB); }
+
+// U|ref>: the reference determinant, then one launch per step.
+int wide_ansatz(qfe_template t, const SlotDesc* d, hipStream_t st) {
+    hipLaunchKernelGGL(k_sec_ref, dim3(wide_blocks(t->D)), dim3(kWideTPB), 0, st, d);
+    QFE_HIP(hipGetLastError());
+    for (uint32_t k = 0; k < (uint32_t)t->step_pairs.size(); ++k) {
+        if (!t->step_pairs[k]) continue;
+        hipLaunchKernelGGL(k_sec_step, dim3(wide_blocks(t->step_pairs[k])), dim3(kWideTPB), 0, st, d, k, 0, 0);
+        QFE_HIP(hipGetLastError());
+    }
+    return QFE_OK;
+}
+
+// Energy: prologue, ansatz, work items, sum (the items and sum kernels of the batched path).
+int wide_energy(qfe_template t, const SlotDesc* d, const uint32_t* dprefix, hipStream_t st) {
+    hipLaunchKernelGGL(k_sec_prologue, dim3(1), dim3(256), 0, st, d, 1);
+    QFE_HIP(hipGetLastError());
+    int rc = wide_ansatz(t, d, st);
+    if (rc) return rc;
+    if (t->nitems) {
+        hipLaunchKernelGGL(k_sec_energy, dim3(t->nitems), dim3(kItemThreads), 0, st, d, dprefix, 1u);
+        QFE_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_sec_sum, dim3(1), dim3(256), 0, st, d);
+    QFE_HIP(hipGetLastError());
+    return QFE_OK;
+}
+
+// psi and sigma = H psi (pool gradients and the gradient below).
+int wide_psi_sigma(qfe_template t, const SlotDesc* d, hipStream_t st) {
+    hipLaunchKernelGGL(k_sec_prologue, dim3(1), dim3(256), 0, st, d, 1);
+    QFE_HIP(hipGetLastError());
+    int rc = wide_ansatz(t, d, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sec_hpsi, dim3(wide_blocks(t->D), 1), dim3(256), 0, st, d);
+    QFE_HIP(hipGetLastError());
+    return QFE_OK;
+}
+
+// Energy and generator overlaps: E = Re<psi|sigma>, then for k = M-1 .. 0 the
+// overlap of generator k and its un-rotation on psi and sigma; fixed-order sums.
+int wide_gradient(qfe_template t, const SlotDesc* d, hipStream_t st) {
+    int rc = wide_psi_sigma(t, d, st);
+    if (rc) return rc;
+    const size_t seg_bytes = align256(t->wide_seg.size() * sizeof(uint2));
+    double* part = (double*)((char*)t->wide + seg_bytes);
+    hipLaunchKernelGGL(k_sec_edot, dim3(t->wide_seg[0].y), dim3(kWideTPB), 0, st, d, part + t->wide_seg[0].x);
+    QFE_HIP(hipGetLastError());
+    for (uint32_t k = (uint32_t)t->ngen; k-- > 0;) {
+        const uint2 sg = t->wide_seg[k + 1];
+        if (!sg.y) continue;
+        const uint32_t s0 = t->h_gen_step[k], s1 = t->h_gen_step[k + 1];
+        if (s1 == s0 + 1) {
+            hipLaunchKernelGGL(k_sec_genrev, dim3(sg.y), dim3(kWideTPB), 0, st, d, s0, part + sg.x);
+            QFE_HIP(hipGetLastError());
+            continue;
+        }
+        hipLaunchKernelGGL(k_sec_genov, dim3(sg.y), dim3(kWideTPB), 0, st, d, s0, s1, part + sg.x);
+        QFE_HIP(hipGetLastError());
+        for (uint32_t sdx = s1; sdx-- > s0;) {
+            if (!t->step_pairs[sdx]) continue;
+            hipLaunchKernelGGL(k_sec_step, dim3(wide_blocks(t->step_pairs[sdx])), dim3(kWideTPB), 0, st, d, sdx, 1, 1);
+            QFE_HIP(hipGetLastError());
+        }
+    }
+    const uint32_t nseg = (uint32_t)t->wide_seg.size();
+    hipLaunchKernelGGL(k_sec_wsum, dim3(wide_blocks(nseg)), dim3(kWideTPB), 0, st, d, (const double*)part,
+                       (const uint2*)t->wide, nseg);
+    QFE_HIP(hipGetLastError());
+    return QFE_OK;
 }
 
 // Single request on the library stream.  Arena: [desc 256][out 256][prefix 256][io][work + gradient parts].
@@ -1042,6 +1284,7 @@ int single_psi_sigma(qfe_template t, const double* params, const double* inputs,
     int rc = single_setup(t, params, inputs, true, S);
     if (rc) return rc;
     hipStream_t st = ctx().stream;
+    if (t->tier == 2) return wide_psi_sigma(t, S.ddesc, st);
     hipLaunchKernelGGL(k_sec_prologue, dim3(1), dim3(256), 0, st, S.ddesc, 1);
     QFE_HIP(hipGetLastError());
     rc = launch_ansatz(t->tier, 1, t->lds_bytes, st, S.ddesc);
@@ -1325,6 +1568,7 @@ int qfe_template_destroy(qfe_template t) {
     if (ctx().ready) (void)hipStreamSynchronize(ctx().stream);
     if (t->dev) (void)hipFree(t->dev);
     if (t->arena) (void)hipFree(t->arena);
+    if (t->wide) (void)hipFree(t->wide);
     delete t;
     return QFE_OK;
 }
@@ -1342,7 +1586,10 @@ int qfe_template_energy(qfe_template t, const double* params, const double* inpu
     Single S;
     int rc = single_setup(t, params, inputs, false, S);
     if (rc) return rc;
-    rc = launch_energy({t}, t->tier, t->lds_bytes, S.ddesc, S.dprefix, t->nitems, ctx().stream);
+    if (t->tier == 2)
+        rc = wide_energy(t, S.ddesc, S.dprefix, ctx().stream);
+    else
+        rc = launch_energy({t}, t->tier, t->lds_bytes, S.ddesc, S.dprefix, t->nitems, ctx().stream);
     if (rc) return rc;
     return fetch_doubles(S.dout, 1, energy);
 }
@@ -1357,7 +1604,7 @@ int qfe_template_prepare(qfe_template t, const double* params, qfe_state out) {
     hipStream_t st = ctx().stream;
     hipLaunchKernelGGL(k_sec_prologue, dim3(1), dim3(256), 0, st, S.ddesc, 0);
     QFE_HIP(hipGetLastError());
-    rc = launch_ansatz(t->tier, 1, t->lds_bytes, st, S.ddesc);
+    rc = t->tier == 2 ? wide_ansatz(t, S.ddesc, st) : launch_ansatz(t->tier, 1, t->lds_bytes, st, S.ddesc);
     if (!rc) rc = launch_zero(out->d, out->N, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_sector_scatter, dim3((unsigned)((t->D + 255) / 256)), dim3(256), 0, st,
@@ -1371,7 +1618,9 @@ int qfe_template_gradient(qfe_template t, const double* params, const double* in
         return fail(QFE_ERR_ARG, "qfe_template_gradient: null argument");
     Single S;
     int rc = single_setup(t, params, inputs, true, S);
-    if (!rc) rc = launch_gradient({t}, t->tier, t->lds_bytes, S.ddesc, ctx().stream);
+    if (!rc)
+        rc = t->tier == 2 ? wide_gradient(t, S.ddesc, ctx().stream)
+                          : launch_gradient({t}, t->tier, t->lds_bytes, S.ddesc, ctx().stream);
     if (rc) return rc;
     std::vector<double> acc(t->ngen + 1);
     rc = fetch_doubles((const double*)(S.work + t->off_gacc), t->ngen, acc.data());

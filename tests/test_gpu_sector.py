@@ -226,3 +226,46 @@ def test_sector_pool_gradients_match_program(qf):
     b = np.array(prog.pool_gradients(t, ops, 64))
     assert a.shape == b.shape
     assert np.max(np.abs(a - b)) < 1e-10
+
+
+def test_large_sector_grid_wide_path_matches_program(qf):
+    """An 18-qubit template (sector too large for one workgroup's LDS: the
+    grid-wide single-request path, one launch per step and per generator)
+    against the full-space program: energy, amplitudes and gradient.  The
+    last generator is two excitations in one operator (two runs: the
+    multi-step branch of the reverse sweep)."""
+    from qforte_amd.engine import workloads as W
+
+    n = 18
+    norb, na, nb, ref = W.half_filled_reference(n)
+    h, g = qf.synthetic_integrals(norb, 31)
+    hmap = qf.HamiltonianMap(norb)
+    inputs = hmap.pack(0.0, h, g)
+    H = hmap.operator(inputs)
+    pool = sd_pool(qf, ref)
+    tops = list(range(0, len(pool), 7))  # a sample of singles and doubles
+    gens, idx, scale = qf.ucc_generators(pool, tops)
+    combo = qf.QubitOperator()
+    combo.add_op(gens[1])
+    combo.add_op(gens[-1])
+    gens = list(gens) + [combo]
+    idx = list(idx) + [len(tops)]
+    scale = list(scale) + [0.5]
+    nparam = len(tops) + 1
+    ridx = qf.ref_to_basis_idx(ref)
+    prog = qf.UCCProgram(n, ridx, gens, idx, scale, nparam, H)
+    tpl = qf.SectorTemplate(n, ridx, gens, idx, scale, nparam, hmap)
+    assert tpl.dim == 126 * 126
+    t = list(np.random.default_rng(3).normal(0.0, 0.1, nparam))
+    assert abs(tpl.energy(t, inputs) - prog.energy(t)) < E_TOL
+    a, b = qf.Computer(n), qf.Computer(n)
+    prog.prepare(t, a)
+    tpl.prepare(t, b)
+    assert np.max(np.abs(np.asarray(a.get_coeff_vec()) - np.asarray(b.get_coeff_vec()))) < 1e-12
+    e, gr = tpl.gradient(t, inputs)
+    e_p, g_p = prog.gradient(t)
+    assert abs(e - e_p) < E_TOL
+    assert np.max(np.abs(np.array(gr) - np.array(g_p))) < 1e-10
+    # repeated calls give the same numbers (fixed-order sums)
+    e2, gr2 = tpl.gradient(t, inputs)
+    assert e2 == e and list(gr2) == list(gr)
