@@ -841,6 +841,11 @@ struct qfe_template_s {
     std::vector<uint2> wide_seg;  // (first partial, count): the energy, then generator k
     uint32_t wide_parts = 0;
     void* wide = nullptr;
+    // captured single-request sequences (energy, gradient) and their pinned
+    // host block: [request head as uploaded][energy][generator overlaps]
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    void* gpin = nullptr;
 };
 
 namespace qfe {
@@ -1294,6 +1299,76 @@ int single_psi_sigma(qfe_template t, const double* params, const double* inputs,
     return QFE_OK;
 }
 
+// ---- captured single-request sequences (large sectors) ----------------------------------
+// The grid-wide sequences launch one kernel per step and per generator; as a
+// hipGraph per template they replay without per-launch host work.  The first
+// node uploads the whole request head (descriptor, prefix, amplitudes,
+// inputs) from the template's pinned block, so a replay does not depend on
+// what other calls left in the arena; the last nodes copy the energy (and the
+// generator overlaps) back into the same block.
+size_t single_head(qfe_template t) { return 768 + align256(t->io_count * 8 + 8); }
+
+int graph_build(qfe_template t, int grad) {
+    const size_t head = single_head(t);
+    if (!t->arena) QFE_HIP(hipMalloc(&t->arena, head + t->grad_bytes));
+    if (!t->gpin) QFE_HIP(hipHostMalloc(&t->gpin, head + 8 * (t->ngen + 2), hipHostMallocDefault));
+    char* dev = (char*)t->arena;
+    char* pin = (char*)t->gpin;
+    const SlotDesc* ddesc = (const SlotDesc*)dev;
+    double* dout = (double*)(dev + 256);
+    // descriptor with the gradient parts (the energy sequence ignores them) and the one-slot prefix
+    SlotDesc d = make_slot(t, dev + head, (const double*)(dev + 768), dout, true);
+    std::memcpy(pin, &d, sizeof(d));
+    const uint32_t pre[2] = {0, t->nitems};
+    std::memcpy(pin + 512, pre, sizeof(pre));
+    hipStream_t st = ctx().stream;
+    QFE_HIP(hipStreamSynchronize(st));
+    QFE_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    auto record = [&]() -> int {
+        QFE_HIP(hipMemcpyAsync(dev, pin, head, hipMemcpyHostToDevice, st));
+        int r = grad ? wide_gradient(t, ddesc, st) : wide_energy(t, ddesc, (const uint32_t*)(dev + 512), st);
+        if (r) return r;
+        QFE_HIP(hipMemcpyAsync(pin + head, dout, 8, hipMemcpyDeviceToHost, st));
+        if (grad && t->ngen)
+            QFE_HIP(hipMemcpyAsync(pin + head + 8, dev + head + t->off_gacc, t->ngen * 8, hipMemcpyDeviceToHost, st));
+        return QFE_OK;
+    };
+    int rc = record();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "qfe_template: end capture");
+    if (!rc) {
+        const hipError_t ie = hipGraphInstantiate(&t->gexec[grad], g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) rc = hip_fail(ie, "qfe_template: instantiate");
+    }
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        t->gexec[grad] = nullptr;
+        return rc;
+    }
+    t->graph[grad] = g;
+    return QFE_OK;
+}
+
+// Replay: energy into *energy, and with grad the ngen generator overlaps into acc.
+int graph_run(qfe_template t, int grad, const double* params, const double* inputs, double* energy, double* acc) {
+    if (!t->gexec[grad]) {
+        int rc = graph_build(t, grad);
+        if (rc) return rc;
+    }
+    char* pin = (char*)t->gpin;
+    const size_t head = single_head(t);
+    // every call synchronizes before returning, so no replay is reading the block
+    if (t->nparam) std::memcpy(pin + 768, params, t->nparam * 8);
+    if (t->ninputs) std::memcpy(pin + 768 + t->nparam * 8, inputs, t->ninputs * 8);
+    hipStream_t st = ctx().stream;
+    QFE_HIP(hipGraphLaunch(t->gexec[grad], st));
+    QFE_HIP(hipStreamSynchronize(st));
+    std::memcpy(energy, pin + head, 8);
+    if (grad && t->ngen) std::memcpy(acc, pin + head + 8, t->ngen * 8);
+    return QFE_OK;
+}
+
 }  // namespace
 }  // namespace qfe
 
@@ -1569,6 +1644,11 @@ int qfe_template_destroy(qfe_template t) {
     if (t->dev) (void)hipFree(t->dev);
     if (t->arena) (void)hipFree(t->arena);
     if (t->wide) (void)hipFree(t->wide);
+    for (int k = 0; k < 2; ++k) {
+        if (t->gexec[k]) (void)hipGraphExecDestroy(t->gexec[k]);
+        if (t->graph[k]) (void)hipGraphDestroy(t->graph[k]);
+    }
+    if (t->gpin) (void)hipHostFree(t->gpin);
     delete t;
     return QFE_OK;
 }
@@ -1583,13 +1663,11 @@ int qfe_template_info(qfe_template t, uint64_t* sector_dim, uint64_t* request_by
 int qfe_template_energy(qfe_template t, const double* params, const double* inputs, double* energy) {
     if (!t || !energy || (t->nparam && !params) || (t->ninputs && !inputs))
         return fail(QFE_ERR_ARG, "qfe_template_energy: null argument");
+    if (t->tier == 2) return graph_run(t, 0, params, inputs, energy, nullptr);
     Single S;
     int rc = single_setup(t, params, inputs, false, S);
     if (rc) return rc;
-    if (t->tier == 2)
-        rc = wide_energy(t, S.ddesc, S.dprefix, ctx().stream);
-    else
-        rc = launch_energy({t}, t->tier, t->lds_bytes, S.ddesc, S.dprefix, t->nitems, ctx().stream);
+    rc = launch_energy({t}, t->tier, t->lds_bytes, S.ddesc, S.dprefix, t->nitems, ctx().stream);
     if (rc) return rc;
     return fetch_doubles(S.dout, 1, energy);
 }
@@ -1616,13 +1694,18 @@ int qfe_template_prepare(qfe_template t, const double* params, qfe_state out) {
 int qfe_template_gradient(qfe_template t, const double* params, const double* inputs, double* grad, double* energy) {
     if (!t || !grad || (t->nparam && !params) || (t->ninputs && !inputs))
         return fail(QFE_ERR_ARG, "qfe_template_gradient: null argument");
+    std::vector<double> acc(t->ngen + 1);
+    if (t->tier == 2) {
+        int rc = graph_run(t, 1, params, inputs, &acc[t->ngen], acc.data());
+        if (rc) return rc;
+        gradient_from_overlaps(t, acc.data(), grad);
+        if (energy) *energy = acc[t->ngen];
+        return QFE_OK;
+    }
     Single S;
     int rc = single_setup(t, params, inputs, true, S);
-    if (!rc)
-        rc = t->tier == 2 ? wide_gradient(t, S.ddesc, ctx().stream)
-                          : launch_gradient({t}, t->tier, t->lds_bytes, S.ddesc, ctx().stream);
+    if (!rc) rc = launch_gradient({t}, t->tier, t->lds_bytes, S.ddesc, ctx().stream);
     if (rc) return rc;
-    std::vector<double> acc(t->ngen + 1);
     rc = fetch_doubles((const double*)(S.work + t->off_gacc), t->ngen, acc.data());
     if (!rc) rc = fetch_doubles(S.dout, 1, &acc[t->ngen]);
     if (rc) return rc;
