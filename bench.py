@@ -33,10 +33,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per k_orbit_lanes<4, 2> launch at 30 qubits from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388756 and 16786200), each
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388759 and 16785998), each
 # scaled by the same counter's reading on k_scale, a plain sweep that reads
-# and writes exactly 16 GiB (factors 2.0000 and 1.0000): 33563712 KiB.
-TRAFFIC_PER_LAUNCH = {30: 33563712 * 1024}
+# and writes exactly 16 GiB (8388621 and 16778273: factors 2.0000 and
+# 0.99994): 33562459 KiB.
+TRAFFIC_PER_LAUNCH = {30: 33562459 * 1024}
 METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
 
 
@@ -83,6 +84,45 @@ def time_single_sweep(qf, W, qc, n, reps=40):
     finally:
         qf.set_tuning(*saved)
     return ms / reps
+
+
+def time_single_gate_sweep(qf, qc, n, reps=40, seed=105):
+    """Average duration of a dense one-qubit gate sweep (Ry then Rz merged
+    into one 2x2 on a random qubit, one gate per pass), HIP events."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    c = qf.Circuit()
+    for _ in range(reps):
+        q = int(rng.integers(n))
+        c.add(qf.gate("Ry", q, q, float(rng.uniform(0.0, 2 * math.pi))))
+        c.add(qf.gate("Rz", q, q, float(rng.uniform(0.0, 2 * math.pi))))
+    saved = qf.get_tuning()
+    qf.set_tuning(0, 0, 1)
+    try:
+        qc.apply_circuit(c)  # warm
+        qf.synchronize()
+        t = qf.DeviceTimer()
+        t.start()
+        qc.apply_circuit(c)
+        ms = t.stop()
+    finally:
+        qf.set_tuning(*saved)
+    return ms / reps
+
+
+def time_attainable_sweep(qf, qc, reps=10):
+    """Average duration of the plain in-place sweep (apply_constant: read and
+    write every amplitude once, no pairing), the attainable streaming rate."""
+    qc.apply_constant(1.0)
+    qf.synchronize()
+    t = qf.DeviceTimer()
+    t.start()
+    for _ in range(reps):
+        qc.apply_constant(1.0)
+    return t.stop() / reps
 
 
 def run_open_loop(qf, classes, reqs, rate, seed):
@@ -503,6 +543,8 @@ def main():
         avg_ms, passes = time_exp_stage(qf, qc, U)
         achieved = 2 * S / (avg_ms * 1e-3) / 1e9
         single_ms = time_single_sweep(qf, W, qc, n)
+        gate_ms = time_single_gate_sweep(qf, qc, n)
+        copy_ms = time_attainable_sweep(qf, qc)
         roofline = {
             "kernel": ("k_orbit_lanes<4, 2> (6-dimension register orbits over 4 lanes of 16 amplitudes; the exponential stage: %d passes "
                        "for %d exponentials, 2S = %.2f GB per launch)" % (passes, len(strings), 2 * S / 1e9)),
@@ -514,7 +556,14 @@ def main():
             "traffic": TRAFFIC_PER_LAUNCH.get(n),
             "avg_launch_ms": round(avg_ms, 4),
             "single_exponential_sweep": {"kernel": "k_pexp", "avg_launch_ms": round(single_ms, 4),
-                                         "achieved": round(2 * S / (single_ms * 1e-3) / 1e9, 1)},
+                                         "achieved": round(2 * S / (single_ms * 1e-3) / 1e9, 1),
+                                         "frac": round(2 * S / (single_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+            "single_gate_sweep": {"kernel": "k_u1 (dense 2x2 on one qubit)", "avg_launch_ms": round(gate_ms, 4),
+                                  "achieved": round(2 * S / (gate_ms * 1e-3) / 1e9, 1),
+                                  "frac": round(2 * S / (gate_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+            "attainable_in_place_sweep": {"kernel": "k_scale (reads and writes every amplitude once)",
+                                          "avg_launch_ms": round(copy_ms, 4),
+                                          "achieved": round(2 * S / (copy_ms * 1e-3) / 1e9, 1)},
         }
 
     cpu = None

@@ -121,10 +121,20 @@ int staging_commit(void* dev, size_t bytes) {
 
 // ---- elementwise kernels ---------------------------------------------------------
 
-__global__ void k_fill_zero(double2* __restrict__ d, uint64_t N) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < N; i += stride) d[i] = make_double2(0.0, 0.0);
+// Streaming elementwise kernels: kStreamItems amplitudes per thread, each
+// instruction of a wave covering 1 KiB contiguous, workgroups spread over the
+// XCDs in contiguous eighths; non-temporal for states beyond the caches.
+constexpr int kStreamTPB = 256;
+constexpr int kStreamItems = 4;
+
+template <bool NT>
+__global__ __launch_bounds__(kStreamTPB) void k_fill_zero(double2* __restrict__ d, uint64_t N) {
+    const uint64_t base = xcd_block() * (kStreamTPB * kStreamItems) + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < kStreamItems; ++u) {
+        const uint64_t i = base + (uint64_t)u * kStreamTPB;
+        if (i < N) st_s<NT>(d + i, make_double2(0.0, 0.0));
+    }
 }
 
 __global__ void k_set_one(double2* __restrict__ d, uint64_t idx) {
@@ -137,17 +147,47 @@ __global__ void k_scatter(double2* __restrict__ d, const uint64_t* __restrict__ 
     if (i < count) d[idx[i]] = v[i];
 }
 
-__global__ void k_scale(double2* __restrict__ d, uint64_t N, double2 a) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < N; i += stride) d[i] = cmul(a, d[i]);
+template <bool NT>
+__global__ __launch_bounds__(kStreamTPB) void k_scale(double2* __restrict__ d, uint64_t N, double2 a) {
+    const uint64_t base = xcd_block() * (kStreamTPB * kStreamItems) + threadIdx.x;
+    double2 v[kStreamItems];
+#pragma unroll
+    for (int u = 0; u < kStreamItems; ++u) {
+        const uint64_t i = base + (uint64_t)u * kStreamTPB;
+        if (i < N) v[u] = ld_s<NT>(d + i);
+    }
+#pragma unroll
+    for (int u = 0; u < kStreamItems; ++u) {
+        const uint64_t i = base + (uint64_t)u * kStreamTPB;
+        if (i < N) st_s<NT>(d + i, cmul(a, v[u]));
+    }
 }
 
-__global__ void k_axpy(double2* __restrict__ y, const double2* __restrict__ x, uint64_t N, double2 a) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < N; i += stride) y[i] = cadd(y[i], cmul(a, x[i]));
+template <bool NT>
+__global__ __launch_bounds__(kStreamTPB) void k_axpy(double2* __restrict__ y, const double2* __restrict__ x, uint64_t N,
+                                                     double2 a) {
+    const uint64_t base = xcd_block() * (kStreamTPB * kStreamItems) + threadIdx.x;
+    double2 v[kStreamItems], w[kStreamItems];
+#pragma unroll
+    for (int u = 0; u < kStreamItems; ++u) {
+        const uint64_t i = base + (uint64_t)u * kStreamTPB;
+        if (i < N) {
+            v[u] = ld_s<NT>(y + i);
+            w[u] = ld_s<NT>(x + i);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kStreamItems; ++u) {
+        const uint64_t i = base + (uint64_t)u * kStreamTPB;
+        if (i < N) st_s<NT>(y + i, cadd(v[u], cmul(a, w[u])));
+    }
 }
+
+static unsigned stream_grid(uint64_t N) {
+    const uint64_t g = (N + kStreamTPB * kStreamItems - 1) / (kStreamTPB * kStreamItems);
+    return (unsigned)(g ? g : 1);
+}
+static bool stream_nt(uint64_t N) { return N && use_nt(63 - __builtin_clzll(N), 63u); }
 
 // Deterministic two-stage reductions.  Stage 1: block b reduces the fixed
 // contiguous slice [b*chunk, (b+1)*chunk) in a fixed order; stage 2: one block
@@ -215,7 +255,10 @@ static unsigned grid_for(uint64_t N, int block = 256, uint64_t cap = 65536) {
 }
 
 int launch_zero(double2* d, uint64_t N, hipStream_t st) {
-    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(N)), dim3(256), 0, st, d, N);
+    if (stream_nt(N))
+        hipLaunchKernelGGL(k_fill_zero<true>, dim3(stream_grid(N)), dim3(kStreamTPB), 0, st, d, N);
+    else
+        hipLaunchKernelGGL(k_fill_zero<false>, dim3(stream_grid(N)), dim3(kStreamTPB), 0, st, d, N);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
 }
@@ -525,7 +568,12 @@ int qfe_state_get_amp(qfe_state s, uint64_t idx, qfe_c64* out) {
 
 int qfe_state_scale(qfe_state s, qfe_c64 a) {
     if (!valid(s)) return fail(QFE_ERR_ARG, "qfe_state_scale: invalid handle");
-    hipLaunchKernelGGL(k_scale, dim3(grid_for(s->N)), dim3(256), 0, g_ctx.stream, s->d, s->N, make_double2(a.re, a.im));
+    if (stream_nt(s->N))
+        hipLaunchKernelGGL(k_scale<true>, dim3(stream_grid(s->N)), dim3(kStreamTPB), 0, g_ctx.stream, s->d, s->N,
+                           make_double2(a.re, a.im));
+    else
+        hipLaunchKernelGGL(k_scale<false>, dim3(stream_grid(s->N)), dim3(kStreamTPB), 0, g_ctx.stream, s->d, s->N,
+                           make_double2(a.re, a.im));
     QFE_HIP(hipGetLastError());
     g_ctx.counters.single++;
     g_ctx.counters.bytes += 32.0 * (double)s->N;
@@ -535,7 +583,12 @@ int qfe_state_scale(qfe_state s, qfe_c64 a) {
 int qfe_state_axpy(qfe_state y, qfe_c64 a, qfe_state x) {
     if (!valid(y) || !valid(x)) return fail(QFE_ERR_ARG, "qfe_state_axpy: invalid handle");
     if (y->n != x->n) return fail(QFE_ERR_ARG, "qfe_state_axpy: qubit counts differ");
-    hipLaunchKernelGGL(k_axpy, dim3(grid_for(y->N)), dim3(256), 0, g_ctx.stream, y->d, x->d, y->N, make_double2(a.re, a.im));
+    if (stream_nt(y->N))
+        hipLaunchKernelGGL(k_axpy<true>, dim3(stream_grid(y->N)), dim3(kStreamTPB), 0, g_ctx.stream, y->d, x->d, y->N,
+                           make_double2(a.re, a.im));
+    else
+        hipLaunchKernelGGL(k_axpy<false>, dim3(stream_grid(y->N)), dim3(kStreamTPB), 0, g_ctx.stream, y->d, x->d, y->N,
+                           make_double2(a.re, a.im));
     QFE_HIP(hipGetLastError());
     g_ctx.counters.single++;
     g_ctx.counters.bytes += 48.0 * (double)y->N;
