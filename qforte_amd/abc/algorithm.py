@@ -2,11 +2,14 @@
 abc/ansatz.py, abc/mixin.py, abc/vqeabc.py).
 
 The drivers keep the reference's constructor options and attribute names.
-Energies and gradients of UCC ansatze go through a compiled program
-(``qfe_program``) -- the same exponentials, term order and Hamiltonian as the
-reference's circuit path, resident in HBM -- unless ``use_program=False``
-selects the gate-level path through ``Computer`` (identical results within
-1e-12; both are tested).
+Energies and gradients of UCC ansatze go through a compiled program -- the
+same exponentials, term order and Hamiltonian as the reference's circuit
+path, resident in HBM.  When every generator keeps the alpha and beta
+particle numbers the program runs on the reference's (N_alpha, N_beta)
+determinant sector (``SectorTemplate``), otherwise, or with
+``use_sector=False``, on the full state (``UCCProgram``); ``use_program=False``
+selects the gate-level path through ``Computer``.  All three agree within
+1e-10 and are tested.
 """
 
 import numpy as np
@@ -14,7 +17,7 @@ import numpy as np
 from .. import _core
 from ..utils.state_prep import build_refprep, ref_string
 from ..utils.trotterization import trotterize
-from ..engine.program import compile_ucc_program
+from ..engine.program import compile_ucc_program, compile_ucc_template
 
 
 class Trotterizable:
@@ -107,7 +110,7 @@ class AnsatzAlgorithm(Algorithm):
     """Operator-pool ansatz machinery (algorithm.py:227-480)."""
 
     def __init__(self, *args, qubit_excitations=False, compact_excitations=False, diis_max_dim=8,
-                 max_moment_rank=0, moment_dt=None, penalty=None, use_program=True, **kwargs):
+                 max_moment_rank=0, moment_dt=None, penalty=None, use_program=True, use_sector=True, **kwargs):
         super().__init__(*args, **kwargs)
         if compact_excitations:
             raise NotImplementedError("compact excitation circuits are not part of this package")
@@ -125,6 +128,7 @@ class AnsatzAlgorithm(Algorithm):
         self._diis_max_dim = diis_max_dim
         self._max_moment_rank = max_moment_rank
         self._use_program = use_program
+        self._use_sector = use_sector
         self._program = None
         self._program_key = None
 
@@ -160,8 +164,16 @@ class AnsatzAlgorithm(Algorithm):
         key = (tuple(self._tops), self._pool_type if isinstance(self._pool_type, str) else id(self._pool_type))
         if self._program is None or self._program_key != key:
             pool_type = self._pool_type if isinstance(self._pool_type, str) else None
-            self._program = compile_ucc_program(self._nqb, self._ref, self._qb_ham, self._pool_obj, list(self._tops),
-                                                pool_type, self._qubit_excitations)
+            args = (self._nqb, self._ref, self._qb_ham, self._pool_obj, list(self._tops), pool_type,
+                    self._qubit_excitations)
+            self._program = None
+            if self._use_sector and self._nqb % 2 == 0:
+                try:
+                    self._program = compile_ucc_template(*args)
+                except ValueError:  # a generator leaves the sector: full state instead
+                    self._program = None
+            if self._program is None:
+                self._program = compile_ucc_program(*args)
             self._program_key = key
         return self._program
 

@@ -32,6 +32,20 @@ def test_he_uccsd_vqe_gate_path(qf):
     assert alg.get_gs_energy() == approx(-2.887594831090938, abs=1.0e-10)
 
 
+def test_he_uccsd_vqe_full_space_program(qf):
+    # the compiled program on the full state instead of the determinant sector
+    alg = qf.UCCNVQE(he(qf), use_sector=False)
+    alg.run(pool_type="SD", use_analytic_grad=True)
+    assert alg.get_gs_energy() == approx(-2.887594831090938, abs=1.0e-10)
+    assert type(alg._get_program()).__name__ == "UCCProgram"
+
+
+def test_default_drivers_run_on_the_sector(qf):
+    alg = qf.UCCNVQE(he(qf))
+    alg.run(pool_type="SD", use_analytic_grad=True)
+    assert type(alg._get_program()).__name__ == "SectorTemplate"
+
+
 def test_he_uccsd_vqe_jacobi(qf):
     # tests/test_uccsd.py:31-47
     alg = qf.UCCNVQE(he(qf))
