@@ -96,7 +96,8 @@ struct SecTables {  // device view of a template (lives in device memory)
     const uint64_t* map_off;
     const uint32_t* map_idx;
     const double2* map_val;
-    uint32_t nsteps, nentries, ngroups, nterms, nitems, ngen;
+    const uint2* hchunks;  // groups [x, y) staged together by the sigma kernel
+    uint32_t nsteps, nentries, ngroups, nterms, nitems, ngen, nhchunks;
 };
 
 struct SlotDesc {
@@ -354,32 +355,97 @@ __global__ __launch_bounds__(256) void k_sec_sum(const SlotDesc* __restrict__ sl
 // ---- gradients ---------------------------------------------------------------------------
 // sigma = H psi, output-centric: each determinant gathers from every group
 // whose flip keeps it in the sector.
+constexpr uint32_t kHpsiGroups = 256;  // group headers staged per chunk
+constexpr uint32_t kHpsiTerms = 1024;  // terms staged per chunk (larger chunks read terms from memory)
+
+// PER determinants per thread: each staged term is read once for all of them
+// (batched launches use 4; a single request keeps 1 so it fills the chip).
+template <int PER>
 __global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ slots) {
+    constexpr int kHpsiPer = PER;
+    constexpr uint32_t kHpsiSpan = 256 * PER;  // determinants per block
+    __shared__ uint4 hdr[kHpsiGroups];  // xa, xb, first term, term count
+    __shared__ uint64_t gx[kHpsiGroups];
+    __shared__ uint64_t tzs[kHpsiTerms];
+    __shared__ double2 cfs[kHpsiTerms];
     const SlotDesc sd = slots[blockIdx.y];
     const SecTables* __restrict__ T = sd.t;
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= T->D) return;
-    const uint32_t ra = (uint32_t)(r / T->Db), rb = (uint32_t)(r % T->Db);
-    const uint32_t sa = T->astr[ra], sb = T->bstr[rb];
-    const uint64_t j = T->afull[ra] | T->bfull[rb];
-    const double2* __restrict__ coef = sd.coef ? sd.coef : T->tcoef;
-    double sr = 0.0, si = 0.0;
-    for (uint32_t g = 0; g < T->ngroups; ++g) {
-        const SecGroup& gr = T->groups[g];
-        if (2 * __popc(sa & gr.xa) != __popc(gr.xa) || 2 * __popc(sb & gr.xb) != __popc(gr.xb)) continue;
-        const uint64_t r2 = (uint64_t)T->arank[sa ^ gr.xa] * T->Db + (uint64_t)T->brank[sb ^ gr.xb];
-        const uint64_t j2 = j ^ gr.x;
-        double tr = 0.0, ti = 0.0;
-        for (uint32_t t = gr.t_first; t < gr.t_first + gr.t_count; ++t) {
-            const double sg = parity_sign(j2 & T->tz[t]);
-            tr += sg * coef[t].x;
-            ti += sg * coef[t].y;
-        }
-        const double2 p = sd.psi[r2];
-        sr += tr * p.x - ti * p.y;
-        si += tr * p.y + ti * p.x;
+    if ((uint64_t)blockIdx.x * kHpsiSpan >= T->D) return;  // the whole block: a smaller request of the launch
+    uint32_t sa[kHpsiPer], sb[kHpsiPer];
+    uint64_t j[kHpsiPer];
+    bool live[kHpsiPer];
+    double sr[kHpsiPer], si[kHpsiPer];
+#pragma unroll
+    for (int q = 0; q < kHpsiPer; ++q) {
+        const uint64_t r = (uint64_t)blockIdx.x * kHpsiSpan + (uint64_t)q * 256 + threadIdx.x;
+        live[q] = r < T->D;  // every thread stays for the staging barriers
+        const uint32_t ra = live[q] ? (uint32_t)(r / T->Db) : 0u, rb = live[q] ? (uint32_t)(r % T->Db) : 0u;
+        sa[q] = T->astr[ra];
+        sb[q] = T->bstr[rb];
+        j[q] = T->afull[ra] | T->bfull[rb];
+        sr[q] = si[q] = 0.0;
     }
-    sd.sig[r] = make_double2(sr, si);
+    const double2* __restrict__ coef = sd.coef ? sd.coef : T->tcoef;
+    for (uint32_t c = 0; c < T->nhchunks; ++c) {
+        const uint2 ch = T->hchunks[c];
+        const uint32_t t0 = T->groups[ch.x].t_first;
+        const uint32_t t1 = T->groups[ch.y - 1].t_first + T->groups[ch.y - 1].t_count;
+        const bool tl = t1 - t0 <= kHpsiTerms;
+        __syncthreads();
+        for (uint32_t g = ch.x + threadIdx.x; g < ch.y; g += blockDim.x) {
+            const SecGroup& gr = T->groups[g];
+            hdr[g - ch.x] = make_uint4(gr.xa, gr.xb, gr.t_first, gr.t_count);
+            gx[g - ch.x] = gr.x;
+        }
+        if (tl)
+            for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                tzs[t - t0] = T->tz[t];
+                cfs[t - t0] = coef[t];
+            }
+        __syncthreads();
+        for (uint32_t g = 0; g < ch.y - ch.x; ++g) {
+            const uint4 h = hdr[g];
+            const int pa = __popc(h.x), pb = __popc(h.y);
+            bool on[kHpsiPer];
+            bool any = false;
+#pragma unroll
+            for (int q = 0; q < kHpsiPer; ++q) {
+                on[q] = live[q] && 2 * __popc(sa[q] & h.x) == pa && 2 * __popc(sb[q] & h.y) == pb;
+                any |= on[q];
+            }
+            if (!any) continue;
+            uint64_t j2[kHpsiPer];
+            double tr[kHpsiPer], ti[kHpsiPer];
+#pragma unroll
+            for (int q = 0; q < kHpsiPer; ++q) {
+                j2[q] = j[q] ^ gx[g];
+                tr[q] = ti[q] = 0.0;
+            }
+            for (uint32_t t = h.z; t < h.z + h.w; ++t) {
+                const uint64_t z = tl ? tzs[t - t0] : T->tz[t];
+                const double2 cf = tl ? cfs[t - t0] : coef[t];
+#pragma unroll
+                for (int q = 0; q < kHpsiPer; ++q) {
+                    const double sg = parity_sign(j2[q] & z);
+                    tr[q] += sg * cf.x;
+                    ti[q] += sg * cf.y;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kHpsiPer; ++q) {
+                if (!on[q]) continue;
+                const uint64_t r2 = (uint64_t)T->arank[sa[q] ^ h.x] * T->Db + (uint64_t)T->brank[sb[q] ^ h.y];
+                const double2 p = sd.psi[r2];
+                sr[q] += tr[q] * p.x - ti[q] * p.y;
+                si[q] += tr[q] * p.y + ti[q] * p.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kHpsiPer; ++q) {
+        const uint64_t r = (uint64_t)blockIdx.x * kHpsiSpan + (uint64_t)q * 256 + threadIdx.x;
+        if (live[q]) sd.sig[r] = make_double2(sr[q], si[q]);
+    }
 }
 
 // Sum over the pairs of entry en of Re(conj(sig) K psi).
@@ -617,6 +683,7 @@ __global__ void k_sector_scatter(const double2* __restrict__ psi, uint64_t D, ui
 // Ansatz launch shapes by size tier: 0 = state in LDS, 256 threads; 1 = state
 // in LDS, 512 threads; 2 = state in device memory, 1024 threads.
 constexpr size_t kTier0Lds = 40 << 10;
+constexpr int kTier2Threads = 1024;
 
 int launch_ansatz(int tier, unsigned nblocks, size_t lds, hipStream_t st, const SlotDesc* d) {
     if (tier == 0)
@@ -624,7 +691,7 @@ int launch_ansatz(int tier, unsigned nblocks, size_t lds, hipStream_t st, const 
     else if (tier == 1)
         hipLaunchKernelGGL((k_sec_ansatz<true, 512>), dim3(nblocks), dim3(512), lds, st, d);
     else
-        hipLaunchKernelGGL((k_sec_ansatz<false, 1024>), dim3(nblocks), dim3(1024), lds, st, d);
+        hipLaunchKernelGGL((k_sec_ansatz<false, kTier2Threads>), dim3(nblocks), dim3(kTier2Threads), lds, st, d);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
 }
@@ -993,6 +1060,16 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     for (size_t k = 0; k < steps.size(); ++k)
         for (uint32_t e = steps[k].e_first; e < steps[k].e_first + steps[k].e_count; ++e)
             t->step_pairs[k] += entries[e].la_len * entries[e].lb_len;
+    // sigma-kernel chunks: up to kHpsiGroups groups and kHpsiTerms terms each
+    std::vector<uint2> hchunks;
+    for (uint32_t g = 0; g < groups.size();) {
+        uint32_t g1 = g + 1;
+        uint32_t terms = groups[g].t_count;
+        while (g1 < groups.size() && g1 - g < kHpsiGroups && terms + groups[g1].t_count <= kHpsiTerms)
+            terms += groups[g1++].t_count;
+        hchunks.push_back(make_uint2(g, g1));
+        g = g1;
+    }
     t->has_map = map_off != nullptr;
     t->nterms = (uint32_t)tz.size();
     t->nentries = (uint32_t)entries.size();
@@ -1040,6 +1117,7 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     const size_t i_mo = pk.add(moff.data(), moff.size() * 8);
     const size_t i_mi = pk.add(midx.data(), midx.size() * 4);
     const size_t i_mv = pk.add(mval.data(), mval.size() * sizeof(double2));
+    const size_t i_hc = pk.add(hchunks.data(), hchunks.size() * sizeof(uint2));
     QFE_HIP(hipMalloc(&t->dev, pk.total));
     char* base = (char*)t->dev;
     auto at = [&](size_t i) { return base + pk.pieces[i].off; };
@@ -1066,6 +1144,8 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     h.map_off = t->has_map ? (const uint64_t*)at(i_mo) : nullptr;
     h.map_idx = (const uint32_t*)at(i_mi);
     h.map_val = (const double2*)at(i_mv);
+    h.hchunks = (const uint2*)at(i_hc);
+    h.nhchunks = (uint32_t)hchunks.size();
     h.nsteps = (uint32_t)steps.size();
     h.nentries = (uint32_t)entries.size();
     h.ngroups = (uint32_t)groups.size();
@@ -1154,12 +1234,12 @@ int launch_energy(const std::vector<qfe_template>& ts, int tier, size_t lds, con
 int launch_gradient(const std::vector<qfe_template>& ts, int tier, size_t lds, const SlotDesc* ddesc, hipStream_t st) {
     const unsigned k = (unsigned)ts.size();
     uint64_t chunks = 1;
-    for (const auto& t : ts) chunks = std::max<uint64_t>(chunks, (t->D + 255) / 256);
+    for (const auto& t : ts) chunks = std::max<uint64_t>(chunks, (t->D + 1023) / 1024);
     hipLaunchKernelGGL(k_sec_prologue, dim3(k), dim3(256), 0, st, ddesc, 1);
     QFE_HIP(hipGetLastError());
     int rc = launch_ansatz(tier, k, lds, st, ddesc);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sec_hpsi, dim3((unsigned)chunks, k), dim3(256), 0, st, ddesc);
+    hipLaunchKernelGGL(k_sec_hpsi<4>, dim3((unsigned)chunks, k), dim3(256), 0, st, ddesc);
     QFE_HIP(hipGetLastError());
     return launch_reverse(tier, k, st, ddesc);
 }
@@ -1200,7 +1280,7 @@ int wide_psi_sigma(qfe_template t, const SlotDesc* d, hipStream_t st) {
     QFE_HIP(hipGetLastError());
     int rc = wide_ansatz(t, d, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sec_hpsi, dim3(wide_blocks(t->D), 1), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_sec_hpsi<1>, dim3((unsigned)((t->D + 255) / 256), 1), dim3(256), 0, st, d);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
 }
@@ -1294,7 +1374,7 @@ int single_psi_sigma(qfe_template t, const double* params, const double* inputs,
     QFE_HIP(hipGetLastError());
     rc = launch_ansatz(t->tier, 1, t->lds_bytes, st, S.ddesc);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sec_hpsi, dim3((unsigned)((t->D + 255) / 256), 1), dim3(256), 0, st, S.ddesc);
+    hipLaunchKernelGGL(k_sec_hpsi<1>, dim3((unsigned)((t->D + 255) / 256), 1), dim3(256), 0, st, S.ddesc);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
 }
