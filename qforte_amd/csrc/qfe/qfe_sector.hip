@@ -188,6 +188,14 @@ __device__ __forceinline__ void pair_rotate(double2* __restrict__ psi, uint64_t 
     psi[r2] = make_double2(c * b.x + s1 * la.x, c * b.y + s1 * la.y);
 }
 
+// <sigma|K|psi> contribution of one pair (the entry_overlap formula).
+__device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, double2 pa, double2 pb, double2 sa,
+                                               double2 sb) {
+    const double s1 = en.w * parity_sign(j1 & en.z0), s2 = en.w * parity_sign((j1 ^ en.x) & en.z0);
+    const double2 k1 = cmul(en.uhi, pb), k2 = cmul(en.ulo, pa);
+    return s2 * (sa.x * k1.x + sa.y * k1.y) + s1 * (sb.x * k2.x + sb.y * k2.y);
+}
+
 // Rotate every pair of entry `en` by (c, sgn * s); pairs of one step are
 // disjoint, so kUnroll of them are in flight per thread (loads first).
 template <int THREADS>
@@ -489,6 +497,31 @@ __global__ __launch_bounds__(THREADS) void k_sec_reverse(const SlotDesc* __restr
     for (uint32_t k = T->ngen; k-- > 0;) {
         const uint32_t s0 = T->gen_step[k], s1 = T->gen_step[k + 1];
         double acc = 0.0;
+        if (s1 == s0 + 1) {
+            // one step: each pair's four old values give its overlap term and its
+            // un-rotated psi and sigma pair in one pass (pairs of a step are disjoint)
+            const uint64_t Db = T->Db;
+            for (uint32_t en = T->steps[s0].e_first; en < T->steps[s0].e_first + T->steps[s0].e_count; ++en) {
+                const SecEntry& E = T->entries[en];
+                const double2 cs = sd.cs[en];
+                const uint32_t lb = E.lb_len, total = E.la_len * lb;
+                for (uint32_t i = tid; i < total; i += THREADS) {
+                    uint32_t ia, ib;
+                    divmod_u(i, lb, E.inv_lb, ia, ib);
+                    const int2 pa = T->lists[E.la_off + ia];
+                    const int2 pb = T->lists[E.lb_off + ib];
+                    const uint64_t r1 = (uint64_t)pa.x * Db + (uint64_t)pb.x, r2 = (uint64_t)pa.y * Db + (uint64_t)pb.y;
+                    const uint64_t j1 = T->afull[pa.x] | T->bfull[pb.x];
+                    const double2 p1 = psi[r1], p2 = psi[r2], g1 = sig[r1], g2 = sig[r2];
+                    acc += pair_overlap(E, j1, p1, p2, g1, g2);
+                    pair_rotate(psi, r1, r2, p1, p2, j1, E.x, E.z0, cs.x, -cs.y, E.ulo, E.uhi);
+                    pair_rotate(sig, r1, r2, g1, g2, j1, E.x, E.z0, cs.x, -cs.y, E.ulo, E.uhi);
+                }
+            }
+            const double2 tot = block_sum(make_double2(acc, 0.0), red);  // its barriers order the writes above
+            if (tid == 0) sd.gacc[k] = tot.x;
+            continue;
+        }
         for (uint32_t s = s0; s < s1; ++s)
             for (uint32_t en = T->steps[s].e_first; en < T->steps[s].e_first + T->steps[s].e_count; ++en)
                 acc += entry_overlap<THREADS>(T, T->entries[en], sig, psi, tid);
@@ -586,13 +619,6 @@ __global__ __launch_bounds__(kWideTPB) void k_sec_step(const SlotDesc* __restric
     if (both) pair_rotate(sd.sig, r1, r2, sd.sig[r1], sd.sig[r2], j1, en.x, en.z0, cs.x, sn, en.ulo, en.uhi);
 }
 
-// <sigma|K|psi> contribution of one pair (the entry_overlap formula).
-__device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, double2 pa, double2 pb, double2 sa,
-                                               double2 sb) {
-    const double s1 = en.w * parity_sign(j1 & en.z0), s2 = en.w * parity_sign((j1 ^ en.x) & en.z0);
-    const double2 k1 = cmul(en.uhi, pb), k2 = cmul(en.ulo, pa);
-    return s2 * (sa.x * k1.x + sa.y * k1.y) + s1 * (sb.x * k2.x + sb.y * k2.y);
-}
 
 // Generator with one step: its overlap and its un-rotation of psi and sigma
 // read the same four old values of each pair, so one pass does both.
