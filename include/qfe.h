@@ -246,8 +246,8 @@ int qfe_template_pool_gradients(qfe_template t, const double* params, const doub
  * No reference counterpart: the reference evaluates one energy per Python
  * call.  Requests (a template, amplitudes and Hamiltonian inputs) queue on
  * the host; each engine step admits queued requests into a launch while the
- * launch's device footprint stays within its share of hbm_budget_bytes and
- * max_active, runs every admitted request (of any template and size) in one
+ * launch's device footprint stays within its share of hbm_budget_bytes
+ * (at most 256 MB per launch, the size of the last-level cache) and max_active, runs every admitted request (of any template and size) in one
  * kernel, one workgroup per request, on one of a few streams, so launches
  * overlap on the device while later arrivals queue for the next launch. */
 int qfe_batch_create(uint64_t hbm_budget_bytes, int max_active, qfe_batch* out);
