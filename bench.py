@@ -395,9 +395,12 @@ def gradient_leg(qf, W, nqubit=20, seed=20, fd_components=6, full_pool_sample=12
     }
 
 
-def cpu_baseline(W, n_cpu, oracle_ref_loader):
+def cpu_baseline(W, n_cpu, oracle_ref_loader, qf=None):
     """Reference Computer (oracle/_ref), one thread, on a bounded sample:
-    1 brickwork layer + 5 exponentials + 5 expectation terms at n_cpu qubits."""
+    1 brickwork layer + 5 exponentials + 5 expectation terms at n_cpu qubits.
+    With ``qf`` the same sample also runs on the GPU (the exponentials as
+    one-pass Pauli exponentials rather than gate ladders) and is compared with
+    the reference: energy and 4096 sampled amplitudes."""
     ref = oracle_ref_loader()
     if ref is None:
         return None
@@ -425,6 +428,22 @@ def cpu_baseline(W, n_cpu, oracle_ref_loader):
     S = 16.0 * (1 << n_cpu)
     # same accounting as the GPU's effective rate: 2S per gate or exponential, S per term
     eff_bytes = 2 * S * (len(layers) + len(strings)) + S * len(terms)
+    parity = None
+    if qf is not None:
+        import numpy as np
+
+        g = qf.Computer(n_cpu)
+        g.apply_circuit(W.to_circuit(qf, layers))
+        U, _ = qf.trotterize(W.exp_operator(qf, strings))
+        g.apply_circuit(U)
+        e_gpu = g.direct_op_exp_val(W.to_operator(qf, terms))
+        amps = np.asarray(g.get_coeff_vec())
+        idx = np.random.default_rng(7).choice(1 << n_cpu, 4096, replace=False)
+        ref_amps = np.array([complex(qc.coeff(ref.QubitBasis(int(i)))) for i in idx])
+        parity = {"energy_abs_diff": float(abs(complex(e) - complex(e_gpu))),
+                  "amplitude_max_abs_diff": float(np.max(np.abs(amps[idx] - ref_amps))),
+                  "amplitudes_compared": len(idx)}
+        del g, amps
     return {
         "seconds": dt,
         "eff_bytes": eff_bytes,
@@ -433,6 +452,7 @@ def cpu_baseline(W, n_cpu, oracle_ref_loader):
         "terms": len(terms),
         "energy": complex(e),
         "reference_gate_sweeps": len(layers) + exp_circ.size() + sum(len(t[1]) + 4 for t in terms),
+        "parity": parity,
     }
 
 
@@ -572,7 +592,7 @@ def main():
         from conftest import load_reference
 
         print(f"[bench] cpu baseline: reference Computer at {args.cpu_nqubit} qubits", file=sys.stderr, flush=True)
-        res = cpu_baseline(W, args.cpu_nqubit, load_reference)
+        res = cpu_baseline(W, args.cpu_nqubit, load_reference, qf)
         if res is not None:
             cpu = {
                 "value": round(res["eff_bytes"] / res["seconds"] / 1e9, 3),
@@ -584,6 +604,7 @@ def main():
                            f"(gate ladders) + {res['terms']}-term expectation, {res['seconds']:.2f} s"),
                 "seconds": round(res["seconds"], 3),
                 "gpu_effective_GBps_same_unit": round(eff_gbps, 1),
+                "parity_vs_gpu": res["parity"],
             }
     serving = None
     if not args.no_serving:
