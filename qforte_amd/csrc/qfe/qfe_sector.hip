@@ -60,7 +60,8 @@ struct SecEntry {  // one class of determinant pairs of a run
     uint32_t la_off, la_len, lb_off, lb_len;
     double w, inv_lb;
     double2 ulo, uhi;  // A(p_lo) / w, A(p_hi) / w
-    uint32_t step, pad;
+    uint32_t step;
+    uint32_t w_off;  // first pair word of the entry (when the template has pair words)
 };
 
 struct SecGroup {  // Hamiltonian terms sharing an X mask, on sector pairs
@@ -97,6 +98,7 @@ struct SecTables {  // device view of a template (lives in device memory)
     const uint32_t* map_idx;
     const double2* map_val;
     const uint2* hchunks;  // groups [x, y) staged together by the sigma kernel
+    const uint2* words;    // pair words of every entry, in entry order (null: use the lists)
     uint32_t nsteps, nentries, ngroups, nterms, nitems, ngen, nhchunks;
 };
 
@@ -179,14 +181,24 @@ __global__ __launch_bounds__(256) void k_sec_prologue(const SlotDesc* __restrict
 // of r1: psi_1' = c a + s s(j2) uhi b, psi_2' = c b + s s(j1) ulo a.  Every
 // kernel that rotates pairs calls this, so a pair's result does not depend on
 // which kernel or launch shape applied it.
-__device__ __forceinline__ void pair_rotate(double2* __restrict__ psi, uint64_t r1, uint64_t r2, double2 a, double2 b,
-                                            uint64_t j1, uint64_t x, uint64_t z0, double c, double s, double2 ulo,
-                                            double2 uhi) {
-    const double s1 = s * parity_sign(j1 & z0), s2 = s * parity_sign((j1 ^ x) & z0);
+__device__ __forceinline__ void pair_rotate_signed(double2* __restrict__ psi, uint64_t r1, uint64_t r2, double2 a,
+                                                   double2 b, double c, double s1, double s2, double2 ulo, double2 uhi) {
     const double2 hb = cmul(uhi, b), la = cmul(ulo, a);
     psi[r1] = make_double2(c * a.x + s2 * hb.x, c * a.y + s2 * hb.y);
     psi[r2] = make_double2(c * b.x + s1 * la.x, c * b.y + s1 * la.y);
 }
+__device__ __forceinline__ void pair_rotate(double2* __restrict__ psi, uint64_t r1, uint64_t r2, double2 a, double2 b,
+                                            uint64_t j1, uint64_t x, uint64_t z0, double c, double s, double2 ulo,
+                                            double2 uhi) {
+    pair_rotate_signed(psi, r1, r2, a, b, c, s * parity_sign(j1 & z0), s * parity_sign((j1 ^ x) & z0), ulo, uhi);
+}
+
+// Pair word of a sector pair: .x = r1 | [parity of (j1 ^ x) & z0] << 31,
+// .y = r2 | [parity of j1 & z0] << 31, j1 the determinant of rank r1.  One
+// 8-byte load per pair instead of two list loads, two string-table loads and
+// a division; ranks stay below 2^31 (at most C(16, 8)^2 determinants).
+constexpr uint32_t kWordRank = 0x7fffffffu;
+__device__ __forceinline__ double word_sign(uint32_t v, double s) { return (v >> 31) ? -s : s; }
 
 // <sigma|K|psi> contribution of one pair (the entry_overlap formula).
 __device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, double2 pa, double2 pb, double2 sa,
@@ -254,7 +266,60 @@ __device__ __forceinline__ void stage_tables(const SecTables* __restrict__ T, ui
 // ---- 2. ansatz: U|ref> by one workgroup per request ------------------------------------
 // With PSI_LDS the state lives in LDS (dynamic LDS = D * 16 bytes + string
 // tables) and is written to the request's buffer at the end.
-template <bool PSI_LDS, int THREADS>
+// Pair words of chunk [base, base + K * THREADS) of an entry with P pairs.
+template <int K, int THREADS>
+__device__ __forceinline__ void load_words(uint2 (&w)[K], const uint2* __restrict__ src, uint32_t P, uint32_t base,
+                                           uint32_t tid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t i = base + k * THREADS + tid;
+        w[k] = i < P ? src[i] : make_uint2(0u, 0u);
+    }
+}
+
+// The entries in order from pair words, K pairs per thread per chunk, with
+// the words of the next chunk (of this entry or the next) loaded while this
+// one is rotated: a step's dependent chain is the amplitude loads and stores.
+template <int K, int THREADS>
+__device__ __forceinline__ void ansatz_words(const SlotDesc& sd, const SecTables* __restrict__ T, double2* __restrict__ psi,
+                                             uint32_t tid) {
+    constexpr uint32_t CH = K * THREADS;
+    const SecEntry* __restrict__ ent = T->entries;
+    const uint2* __restrict__ words = T->words;
+    const uint32_t ne = T->nentries;
+    uint2 w[K];
+    if (ne) load_words<K, THREADS>(w, words + ent[0].w_off, ent[0].la_len * ent[0].lb_len, 0, tid);
+    __syncthreads();
+    for (uint32_t e = 0; e < ne; ++e) {
+        const uint32_t P = ent[e].la_len * ent[e].lb_len, woff = ent[e].w_off;
+        const double2 cs = sd.cs[e], ulo = ent[e].ulo, uhi = ent[e].uhi;
+        for (uint32_t base = 0; base < P; base += CH) {
+            double2 a[K], b[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (base + k * THREADS + tid < P) {
+                    a[k] = psi[w[k].x & kWordRank];
+                    b[k] = psi[w[k].y & kWordRank];
+                }
+            }
+            uint2 nw[K];
+            if (base + CH < P)
+                load_words<K, THREADS>(nw, words + woff, P, base + CH, tid);
+            else if (e + 1 < ne)
+                load_words<K, THREADS>(nw, words + ent[e + 1].w_off, ent[e + 1].la_len * ent[e + 1].lb_len, 0, tid);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (base + k * THREADS + tid < P)
+                    pair_rotate_signed(psi, w[k].x & kWordRank, w[k].y & kWordRank, a[k], b[k], cs.x,
+                                       word_sign(w[k].y, cs.y), word_sign(w[k].x, cs.y), ulo, uhi);
+                w[k] = nw[k];
+            }
+        }
+        if (e + 1 == ne || ent[e + 1].step != ent[e].step) __syncthreads();
+    }
+}
+
+template <bool PSI_LDS, int THREADS, int K>
 __global__ __launch_bounds__(THREADS) void k_sec_ansatz(const SlotDesc* __restrict__ slots) {
     extern __shared__ double2 sh[];
     const SlotDesc sd = slots[blockIdx.x];
@@ -262,17 +327,21 @@ __global__ __launch_bounds__(THREADS) void k_sec_ansatz(const SlotDesc* __restri
     const uint32_t tid = threadIdx.x;
     const uint64_t D = T->D;
     double2* __restrict__ psi = PSI_LDS ? sh : sd.psi;
-    const uint64_t *af, *bf;
-    stage_tables(T, (uint64_t*)(sh + (PSI_LDS ? D : 0)), af, bf, tid, THREADS);
     for (uint64_t r = tid; r < D; r += THREADS) psi[r] = make_double2(r == T->ref_rank ? 1.0 : 0.0, 0.0);
-    __syncthreads();
-    for (uint32_t s = 0; s < T->nsteps; ++s) {
-        const uint32_t e0 = T->steps[s].e_first, e1 = e0 + T->steps[s].e_count;
-        for (uint32_t e = e0; e < e1; ++e) {
-            const double2 cs = sd.cs[e];
-            rotate_entry<THREADS>(T, T->entries[e], cs.x, cs.y, psi, af, bf, tid);
-        }
+    if (T->words) {
+        ansatz_words<K, THREADS>(sd, T, psi, tid);
+    } else {
+        const uint64_t *af, *bf;
+        stage_tables(T, (uint64_t*)(sh + (PSI_LDS ? D : 0)), af, bf, tid, THREADS);
         __syncthreads();
+        for (uint32_t s = 0; s < T->nsteps; ++s) {
+            const uint32_t e0 = T->steps[s].e_first, e1 = e0 + T->steps[s].e_count;
+            for (uint32_t e = e0; e < e1; ++e) {
+                const double2 cs = sd.cs[e];
+                rotate_entry<THREADS>(T, T->entries[e], cs.x, cs.y, psi, af, bf, tid);
+            }
+            __syncthreads();
+        }
     }
     if (PSI_LDS)
         for (uint64_t r = tid; r < D; r += THREADS) sd.psi[r] = psi[r];
@@ -709,15 +778,16 @@ __global__ void k_sector_scatter(const double2* __restrict__ psi, uint64_t D, ui
 // Ansatz launch shapes by size tier: 0 = state in LDS, 256 threads; 1 = state
 // in LDS, 512 threads; 2 = state in device memory, 1024 threads.
 constexpr size_t kTier0Lds = 40 << 10;
+constexpr uint64_t kMaxPairWords = 1ull << 25;  // 256 MB of pair words per template at most
 constexpr int kTier2Threads = 1024;
 
 int launch_ansatz(int tier, unsigned nblocks, size_t lds, hipStream_t st, const SlotDesc* d) {
     if (tier == 0)
-        hipLaunchKernelGGL((k_sec_ansatz<true, 256>), dim3(nblocks), dim3(256), lds, st, d);
+        hipLaunchKernelGGL((k_sec_ansatz<true, 256, 4>), dim3(nblocks), dim3(256), lds, st, d);
     else if (tier == 1)
-        hipLaunchKernelGGL((k_sec_ansatz<true, 512>), dim3(nblocks), dim3(512), lds, st, d);
+        hipLaunchKernelGGL((k_sec_ansatz<true, 512, 4>), dim3(nblocks), dim3(512), lds, st, d);
     else
-        hipLaunchKernelGGL((k_sec_ansatz<false, kTier2Threads>), dim3(nblocks), dim3(kTier2Threads), lds, st, d);
+        hipLaunchKernelGGL((k_sec_ansatz<false, kTier2Threads, 5>), dim3(nblocks), dim3(kTier2Threads), lds, st, d);
     QFE_HIP(hipGetLastError());
     return QFE_OK;
 }
@@ -1127,6 +1197,29 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     std::vector<uint64_t> af(t->Da), bf(t->Db);
     for (uint32_t a = 0; a < t->Da; ++a) af[a] = spread(sp.str[0][a], 0, norb);
     for (uint32_t b2 = 0; b2 < t->Db; ++b2) bf[b2] = spread(sp.str[1][b2], 1, norb);
+    // pair words of every entry, in entry order, when they fit the cap
+    std::vector<uint2> words;
+    {
+        uint64_t total = 0;
+        for (const auto& en : entries) total += (uint64_t)en.la_len * en.lb_len;
+        if (total <= kMaxPairWords) {
+            words.reserve(total);
+            for (auto& en : entries) {
+                en.w_off = (uint32_t)words.size();
+                for (uint32_t ia = 0; ia < en.la_len; ++ia) {
+                    const int2 pa = lp.pool[en.la_off + ia];
+                    for (uint32_t ib = 0; ib < en.lb_len; ++ib) {
+                        const int2 pb = lp.pool[en.lb_off + ib];
+                        const uint64_t j1 = af[pa.x] | bf[pb.x];
+                        const uint32_t r1 = (uint32_t)((uint64_t)pa.x * t->Db + pb.x);
+                        const uint32_t r2 = (uint32_t)((uint64_t)pa.y * t->Db + pb.y);
+                        const uint32_t g1 = popc64(j1 & en.z0) & 1, g2 = popc64((j1 ^ en.x) & en.z0) & 1;
+                        words.push_back(make_uint2(r1 | (g2 << 31), r2 | (g1 << 31)));
+                    }
+                }
+            }
+        }
+    }
     Packer pk(sizeof(SecTables));
     const size_t i_af = pk.add(af.data(), af.size() * 8), i_bf = pk.add(bf.data(), bf.size() * 8);
     const size_t i_as = pk.add(sp.str[0].data(), sp.str[0].size() * 4), i_bs = pk.add(sp.str[1].data(), sp.str[1].size() * 4);
@@ -1144,6 +1237,7 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     const size_t i_mi = pk.add(midx.data(), midx.size() * 4);
     const size_t i_mv = pk.add(mval.data(), mval.size() * sizeof(double2));
     const size_t i_hc = pk.add(hchunks.data(), hchunks.size() * sizeof(uint2));
+    const size_t i_w = pk.add(words.data(), words.size() * sizeof(uint2));
     QFE_HIP(hipMalloc(&t->dev, pk.total));
     char* base = (char*)t->dev;
     auto at = [&](size_t i) { return base + pk.pieces[i].off; };
@@ -1172,6 +1266,7 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     h.map_val = (const double2*)at(i_mv);
     h.hchunks = (const uint2*)at(i_hc);
     h.nhchunks = (uint32_t)hchunks.size();
+    h.words = entries.empty() || words.empty() ? nullptr : (const uint2*)at(i_w);
     h.nsteps = (uint32_t)steps.size();
     h.nentries = (uint32_t)entries.size();
     h.ngroups = (uint32_t)groups.size();
