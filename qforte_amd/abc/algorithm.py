@@ -9,7 +9,9 @@ particle numbers the program runs on the reference's (N_alpha, N_beta)
 determinant sector (``SectorTemplate``), otherwise, or with
 ``use_sector=False``, on the full state (``UCCProgram``); ``use_program=False``
 selects the gate-level path through ``Computer``.  All three agree within
-1e-10 and are tested.
+1e-10 and are tested.  With ``compact_excitations=True`` the circuits are the
+reference's compact excitation circuits, each equal to exp(t kappa), so the
+compiled program serves them too; the circuits fix the reported CNOT counts.
 """
 
 import numpy as np
@@ -17,6 +19,7 @@ import numpy as np
 from .. import _core
 from ..utils.state_prep import build_refprep, ref_string
 from ..utils.trotterization import trotterize
+from ..utils.compact_excitation_circuits import compact_excitation_circuit
 from ..engine.program import compile_ucc_program, compile_ucc_template
 
 
@@ -112,8 +115,6 @@ class AnsatzAlgorithm(Algorithm):
     def __init__(self, *args, qubit_excitations=False, compact_excitations=False, diis_max_dim=8,
                  max_moment_rank=0, moment_dt=None, penalty=None, use_program=True, use_sector=True, **kwargs):
         super().__init__(*args, **kwargs)
-        if compact_excitations:
-            raise NotImplementedError("compact excitation circuits are not part of this package")
         if max_moment_rank:
             raise NotImplementedError("moment energy corrections are not part of this package")
         if penalty is not None:
@@ -157,8 +158,10 @@ class AnsatzAlgorithm(Algorithm):
 
     # --- compiled program -------------------------------------------------------------
     def _can_use_program(self):
+        # A compact excitation circuit equals exp(t kappa) exactly, as the
+        # program applies it, whatever the Trotter number.
         return (self._use_program and self._fast and self._state_prep_type == "occupation_list"
-                and self._trotter_number == 1 and not self._compact_excitations)
+                and (self._trotter_number == 1 or self._compact_excitations))
 
     def _get_program(self):
         key = (tuple(self._tops), self._pool_type if isinstance(self._pool_type, str) else id(self._pool_type))
@@ -221,6 +224,12 @@ class UCC(Trotterizable):
         else:
             for tamp, top in zip(tamps, self._tops):
                 temp_pool.add(tamp, self._pool_obj[top][1])
+        if self._compact_excitations:  # ansatz.py:60-71
+            U = _core.Circuit()
+            for tamp, sq_op in temp_pool:
+                coeff, cre, ann = sq_op.terms()[1]
+                U.add(compact_excitation_circuit(tamp * coeff, cre, ann, self._qubit_excitations))
+            return U
         A = temp_pool.get_qubit_operator("commuting_grp_lex", qubit_excitations=self._qubit_excitations)
         U, phase1 = trotterize(A, trotter_number=self._trotter_number)
         if phase1 != 1.0 + 0.0j:

@@ -4,6 +4,7 @@ import numpy as np
 
 from .. import _core
 from ..utils.trotterization import trotterize
+from ..utils.compact_excitation_circuits import compact_excitation_circuit
 from .algorithm import AnsatzAlgorithm, UCC
 
 
@@ -44,7 +45,18 @@ class UCCVQE(UCC, AnsatzAlgorithm):
         for mu in reversed(range(M - 1)):
             K = self._pool_obj[self._tops[mu]][1].jw_transform(self._qubit_excitations)
             K.mult_coeffs(self._pool_obj[self._tops[mu]][0])
-            if self._pool_type == "sa_SD":
+            if self._compact_excitations:
+                # exp(-t kappa) of the next operator; for sa_SD the de-excitation
+                # form carries the minus sign (uccvqeabc.py:181-207)
+                Umu = _core.Circuit()
+                terms = self._pool_obj[self._tops[mu + 1]][1].terms()
+                if self._pool_type == "sa_SD":
+                    for coeff, cr, ann in terms[: len(terms) // 2]:
+                        Umu.add(compact_excitation_circuit(amps[mu + 1] * coeff, ann, cr, self._qubit_excitations))
+                else:
+                    coeff, cr, ann = terms[1]
+                    Umu.add(compact_excitation_circuit(-amps[mu + 1] * coeff, cr, ann, self._qubit_excitations))
+            elif self._pool_type == "sa_SD":
                 Umu = _core.Circuit()
                 terms = self._pool_obj[self._tops[mu + 1]][1].terms()
                 for coeff, cr, ann in terms[: len(terms) // 2]:

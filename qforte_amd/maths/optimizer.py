@@ -1,9 +1,11 @@
-"""Quasi-Newton (Jacobi) amplitude solver with DIIS acceleration (reference
-src/qforte/maths/optimizer.py:6-116)."""
+"""Quasi-Newton (Jacobi) amplitude solver with DIIS acceleration and the
+scipy minimisation of projective residuals (reference
+src/qforte/maths/optimizer.py:6-170)."""
 
 import copy
 
 import numpy as np
+from scipy.optimize import minimize
 
 
 def diis(diis_max_dim, t_diis, e_diis):
@@ -65,3 +67,35 @@ def jacobi_solver(self):
         self._n_cnot_lst.append(self._n_cnot)
     if hasattr(self, "_n_pauli_trm_measures_lst"):
         self._n_pauli_trm_measures_lst.append(self._Nl * k * (2 * len(self._tamps) + 1))
+
+
+def scipy_solver(self, function_to_minimize):
+    """scipy.optimize.minimize of ``function_to_minimize`` (the squared
+    residual norm for PQE) with the reference's per-method tolerances and
+    bookkeeping (optimizer.py:119-170)."""
+    method = self._optimizer.lower()
+    opts = {"disp": False, "maxiter": self._opt_maxiter}
+    if method in ("bfgs", "cg", "l-bfgs-b"):
+        opts["gtol"] = self._opt_thresh
+    if method == "nelder-mead":
+        opts["fatol"] = self._opt_thresh
+        opts["adaptive"] = True
+    if method in ("powell", "l-bfgs-b", "slsqp"):
+        opts["ftol"] = self._opt_thresh
+    x0 = copy.deepcopy(self._tamps)
+    self._prev_energy = self.energy_feval(x0)
+    self._k_counter = 0
+    res = minimize(function_to_minimize, x0, method=self._optimizer, options=opts, callback=self.report_iteration)
+    if not res.success:
+        print("  => WARNING: minimization result may not be tightly converged.")
+    self._tamps = list(res.x)
+    self._Egs = self.energy_feval(self._tamps)
+    if hasattr(self, "_energies"):
+        self._energies.append(self._Egs)
+    self._n_classical_params = len(self._tamps)
+    if hasattr(self, "_n_pauli_trm_measures"):
+        self._n_pauli_trm_measures = ((self._n_pauli_trm_measures or 0) + 2 * self._Nl * self._k_counter *
+                                      len(self._tamps) + self._Nl * self._k_counter)
+    self._n_cnot = self.build_Uvqc().get_num_cnots()
+    if hasattr(self, "_n_cnot_lst"):
+        self._n_cnot_lst.append(self._n_cnot)

@@ -38,9 +38,28 @@ class UCCNPQE(UCC, AnsatzAlgorithm):
         print(f"Final UCCN-PQE Energy:                       {round(self._Egs, 10)}")
 
     def solve(self):
+        """Jacobi iterations, or scipy minimisation of sum_mu r_mu^2 (uccpqeabc.py:101-122)."""
         if self._optimizer.lower() == "jacobi":
             return optimizer.jacobi_solver(self)
-        raise NotImplementedError("only the jacobi solver is provided for PQE")
+        if self._optimizer.lower() in ("nelder-mead", "powell", "bfgs", "l-bfgs-b", "cg", "slsqp"):
+            return optimizer.scipy_solver(self, self.get_sum_residual_square)
+        raise NotImplementedError(
+            "Currently only Jacobi, Nelder-Mead, Powell, BFGS, L-BFGS-B, CG, and SLSQP solvers are implemented")
+
+    def get_sum_residual_square(self, tamps):
+        r = np.asarray(self.get_residual_vector(tamps))
+        return float(np.sum(r * r))
+
+    def report_iteration(self, x):
+        """scipy callback: one line per iteration (uccpqeabc.py:63-99)."""
+        self._k_counter += 1
+        if self._k_counter == 1:
+            print("\n    k iteration         Energy               dE           Nrvec ev      Nrm ev*         ||r||")
+        self._curr_energy = self.energy_feval(x)
+        dE = self._curr_energy - self._prev_energy
+        print(f"     {self._k_counter:7}        {self._curr_energy:+12.10f}      {dE:+12.10f}      "
+              f"{self._res_vec_evals:4}        {self._res_m_evals:6}       {self._res_vec_norm:+12.10f}")
+        self._prev_energy = self._curr_energy
 
     def fill_excited_dets(self):
         """Excited determinant and phase of every pool operator (uccnpqe.py:193-242)."""

@@ -119,3 +119,67 @@ def test_h2_qite(qf):
     alg = qf.QITE(mol, reference=[1, 1, 0, 0])
     alg.run(beta=18.0, do_lanczos=True, lanczos_gap=49)
     assert alg.get_gs_energy() == approx(-0.9981493534, abs=1.0e-10)
+
+
+def h4(qf):
+    return qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+
+
+@pytest.mark.parametrize("qubit", [False, True])
+def test_compact_excitations_match_standard_circuits(qf, qubit):
+    # tests/test_feb_qeb_circuits.py:6-62 on H4/STO-6G: the same energies with
+    # fewer CNOTs.  The VQE drivers run both forms through the compiled
+    # program (each compact circuit is exp(t kappa)); PQE residuals apply the
+    # compact gate circuits themselves.
+    mol = h4(qf)
+    cases = [(qf.UCCNVQE, {"pool_type": "SD"}), (qf.UCCNVQE, {"pool_type": "sa_SD"}),
+             (qf.ADAPTVQE, {"pool_type": "SD", "avqe_thresh": 1.0e-3}), (qf.UCCNPQE, {"pool_type": "SD"})]
+    for method, options in cases:
+        kw = {"print_summary_file": False} if method is qf.ADAPTVQE else {}
+        std = method(mol, compact_excitations=False, qubit_excitations=qubit, **kw)
+        std.run(**options)
+        eff = method(mol, compact_excitations=True, qubit_excitations=qubit, **kw)
+        eff.run(**options)
+        assert std.get_gs_energy() == approx(eff.get_gs_energy(), abs=1.0e-10), (method.__name__, options)
+        assert std._n_cnot > eff._n_cnot
+
+
+def test_compact_excitations_on_the_gate_path(qf):
+    # the compact circuits and the compact reverse sweep of the analytic
+    # gradient (uccvqeabc.py:181-207) through Computer
+    mol = h4(qf)
+    for pool_type in ("SD", "sa_SD"):
+        std = qf.UCCNVQE(mol)
+        std.run(pool_type=pool_type, use_analytic_grad=True)
+        eff = qf.UCCNVQE(mol, compact_excitations=True, use_program=False)
+        eff.run(pool_type=pool_type, use_analytic_grad=True)
+        assert eff.get_gs_energy() == approx(std.get_gs_energy(), abs=1.0e-9)
+    # one gradient of each pool: the compact reverse sweep equals the standard
+    # gate-level recursion; for SD both are the exact derivative the program
+    # returns.  (For sa_SD the reference's recursion is not the exact
+    # derivative: its spin-adapted halves do not commute.)
+    for pool_type in ("SD", "sa_SD"):
+        std = qf.UCCNVQE(mol, use_program=False)
+        eff = qf.UCCNVQE(mol, compact_excitations=True, use_program=False)
+        for alg in (std, eff):
+            alg._pool_type = pool_type
+            alg.fill_pool()
+            alg._tops = list(range(len(alg._pool_obj)))
+            alg._tamps = [0.05 * (k % 5 - 2) for k in range(len(alg._tops))]
+        g_std = np.asarray(std.measure_gradient())
+        g_eff = np.asarray(eff.measure_gradient())
+        assert np.abs(g_std - g_eff).max() < 1.0e-12, pool_type
+        if pool_type == "SD":
+            g_prog = np.asarray(eff._get_program().gradient(list(eff._tamps))[1])
+            assert np.abs(g_prog - g_eff).max() < 1.0e-12
+
+
+def test_compact_pqe_jacobi_and_bfgs_agree(qf):
+    # tests/test_uccsd.py:86-122: compact-excitation PQE by Jacobi and by BFGS
+    # on the squared residuals reach the same energy
+    mol = h4(qf)
+    jacobi = qf.UCCNPQE(mol, compact_excitations=True, qubit_excitations=False, diis_max_dim=8)
+    jacobi.run(optimizer="jacobi", pool_type="SD")
+    bfgs = qf.UCCNPQE(mol, compact_excitations=True, qubit_excitations=False)
+    bfgs.run(optimizer="BFGS", pool_type="SD")
+    assert jacobi.get_gs_energy() == approx(bfgs.get_gs_energy(), abs=1.0e-8)

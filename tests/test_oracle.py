@@ -3,12 +3,15 @@ against it: known-answer vectors from the reference's tests, brute-force
 Kronecker products, closed forms, and the compiled reference core."""
 
 import math
+import os
 
 import numpy as np
 import pytest
 import scipy.linalg
 
 from qforte_amd.engine import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 rng = np.random.default_rng(1234)
 
@@ -217,3 +220,59 @@ def test_exponential_ladder_matches_reference(oracle, ref):
         qc.apply_circuit(circ)
         psi = oracle.exp_pauli(psi, -1j * theta, term)
     assert np.max(np.abs(np.array(qc.get_coeff_vec()) - psi)) < 1e-13
+
+
+def test_sa_sd_reference_recursion_is_not_the_derivative(oracle, qf):
+    # uccvqeabc.py:151-244 applies the spin-adapted halves of operator mu + 1
+    # in forward order to undo them and measures K_mu after all halves of mu.
+    # The halves of a spin-adapted double do not commute, so for sa_SD pools
+    # the recursion differs from dE/dt; for SD pools it is the derivative.
+    # The compiled program returns dE/dt; the gate-level path keeps the
+    # recursion.
+    import scipy.linalg as sla
+    mol = qf.system_factory(build_type="external", filename=os.path.join(GOLDEN, "H4-sto6g-075a.json"))
+    n = len(mol.hf_reference)
+    H = oracle.operator_matrix(n, [(c, [(g.gate_id(), g.target()) for g in circ.gates()])
+                                   for c, circ in mol.hamiltonian.terms()])
+    ref = np.zeros(1 << n, dtype=complex)
+    ref[sum(1 << i for i, b in enumerate(mol.hf_reference) if b)] = 1.0
+    gaps = {}
+    for pool_type in ("SD", "sa_SD"):
+        pool = qf.SQOpPool()
+        pool.set_orb_spaces(mol.hf_reference)
+        pool.fill_pool(pool_type)
+        halves, full = [], []
+        for coeff, op in pool:
+            terms = op.terms()
+            if pool_type == "sa_SD":
+                halves.append([oracle.jw_dense([(c, cr, an), (-c, an, cr)], n) for c, cr, an in terms[: len(terms) // 2]])
+            else:
+                halves.append([oracle.jw_dense(terms, n)])
+            full.append(coeff * oracle.jw_dense(terms, n))
+        t = np.array([0.05 * (k % 5 - 2) for k in range(len(halves))])
+        U0 = [[sla.expm(tk * K) for K in hs] for tk, hs in zip(t, halves)]
+
+        def energy(k, dt):
+            psi = ref.copy()
+            for j, hs in enumerate(halves):
+                for i, K in enumerate(hs):
+                    psi = (sla.expm((t[j] + dt) * K) if j == k else U0[j][i]) @ psi
+            return float(np.real(np.vdot(psi, H @ psi)))
+
+        h = 1e-6
+        exact = np.array([(energy(k, h) - energy(k, -h)) / (2 * h) for k in range(len(t))])
+        psi = ref.copy()
+        for us in U0:
+            for U in us:
+                psi = U @ psi
+        sig = H @ psi
+        rec = np.zeros(len(t))
+        rec[-1] = 2 * np.real(np.vdot(sig, full[-1] @ psi))
+        for mu in reversed(range(len(t) - 1)):
+            for K in halves[mu + 1]:
+                U = sla.expm(-t[mu + 1] * K)
+                sig, psi = U @ sig, U @ psi
+            rec[mu] = 2 * np.real(np.vdot(sig, full[mu] @ psi))
+        gaps[pool_type] = np.abs(rec - exact).max()
+    assert gaps["SD"] < 1e-8
+    assert gaps["sa_SD"] > 1e-4
