@@ -131,6 +131,14 @@ int qfe_state_equal(qfe_state a, qfe_state b, int* equal);
 /* Probability scan sum |c_i|^2 (norm squared); no direct reference counterpart,
  * used by norm checks. */
 int qfe_state_norm2(qfe_state s, double* out);
+/* Draw nsamples computational-basis indices from |psi|^2: the probability
+ * scan and std::discrete_distribution draw of Computer::measure_circuit,
+ * measure_z_readouts_fast and measure_readouts (computer.cc:166-188,
+ * 195-210, 247-262; the basis change before it is an ordinary apply on a
+ * copy).  A counter-based generator seeded by `seed` draws the uniforms, so a
+ * seed repeats its indices (the reference seeds from std::random_device).
+ * out: nsamples indices in host memory; at most 2^32 per call. */
+int qfe_sample(qfe_state s, uint64_t nsamples, uint64_t seed, uint64_t* out);
 
 /* ---- gates and Pauli exponentials --------------------------------------- */
 /* apply_gate / apply_circuit (computer.cc:93-114, 340-751).  Gates are fused

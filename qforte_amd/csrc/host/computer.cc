@@ -331,7 +331,7 @@ cplx Computer::direct_pauli_circ_exp_val(const Circuit& c) {
     return direct_circ_exp_val(c);
 }
 
-std::vector<cplx> Computer::basis_rotated_probs(const Circuit& c) {
+Computer Computer::basis_rotated(const Circuit& c) const {
     // measurement basis change of computer.cc:141-165: Z -> I, X -> H, Y -> Rx(pi/2)
     Computer tmp(*this);
     Circuit rot;
@@ -343,65 +343,57 @@ std::vector<cplx> Computer::basis_rotated_probs(const Circuit& c) {
         else if (id == "Y") rot.add_gate(make_gate("Rx", t, t, M_PI / 2));
     }
     tmp.apply_circuit(rot);
-    return tmp.get_coeff_vec();
+    return tmp;
+}
+
+// Basis indices drawn from |psi|^2 on the device, seeded from
+// std::random_device as the reference's generator is (computer.cc:169-174).
+std::vector<uint64_t> Computer::sample(size_t n_measurements) const {
+    std::random_device rd;
+    const uint64_t seed = ((uint64_t)rd() << 32) ^ (uint64_t)rd();
+    std::vector<uint64_t> idx(n_measurements);
+    check(qfe_sample(s_, n_measurements, seed, idx.data()), "measure");
+    return idx;
 }
 
 std::vector<double> Computer::measure_circuit(const Circuit& c, size_t n_measurements) {
-    std::vector<cplx> a = basis_rotated_probs(c);
-    std::vector<double> p(a.size());
-    for (size_t k = 0; k < a.size(); ++k) p[k] = std::norm(a[k]);
-    std::random_device rd;
-    std::mt19937 gen(rd());
-    std::discrete_distribution<> dd(p.begin(), p.end());
+    const std::vector<uint64_t> idx = basis_rotated(c).sample(n_measurements);
     std::vector<double> out(n_measurements);
     for (size_t k = 0; k < n_measurements; ++k) {
-        const uint64_t m = (uint64_t)dd(gen);
         double v = 1.0;
-        for (const auto& g : c.gates()) v *= 1.0 - 2.0 * (double)((m >> g.target()) & 1ull);
+        for (const auto& g : c.gates()) v *= 1.0 - 2.0 * (double)((idx[k] >> g.target()) & 1ull);
         out[k] = v;
     }
     return out;
 }
 
 std::vector<std::vector<int>> Computer::measure_z_readouts_fast(size_t na, size_t nb, size_t n_measurements) {
-    std::vector<cplx> a = get_coeff_vec();
-    std::vector<double> p(a.size());
-    for (size_t k = 0; k < a.size(); ++k) p[k] = std::norm(a[k]);
-    std::random_device rd;
-    std::mt19937 gen(rd());
-    std::discrete_distribution<> dd(p.begin(), p.end());
+    const std::vector<uint64_t> idx = sample(n_measurements);
     std::vector<std::vector<int>> out(n_measurements);
-    for (size_t k = 0; k < n_measurements; ++k) {
-        const uint64_t m = (uint64_t)dd(gen);
-        for (size_t l = na; l < nb + 1; ++l) out[k].push_back((int)((m >> l) & 1ull));
-    }
+    for (size_t k = 0; k < n_measurements; ++k)
+        for (size_t l = na; l < nb + 1; ++l) out[k].push_back((int)((idx[k] >> l) & 1ull));
     return out;
 }
 
 std::vector<std::vector<int>> Computer::measure_readouts(const Circuit& c, size_t n_measurements) {
-    std::vector<cplx> a = basis_rotated_probs(c);
-    std::vector<double> p(a.size());
-    for (size_t k = 0; k < a.size(); ++k) p[k] = std::norm(a[k]);
-    std::random_device rd;
-    std::mt19937 gen(rd());
-    std::discrete_distribution<> dd(p.begin(), p.end());
+    const std::vector<uint64_t> idx = basis_rotated(c).sample(n_measurements);
     std::vector<std::vector<int>> out(n_measurements);
-    for (size_t k = 0; k < n_measurements; ++k) {
-        const uint64_t m = (uint64_t)dd(gen);
-        for (const auto& g : c.gates()) out[k].push_back((int)((m >> g.target()) & 1ull));
-    }
+    for (size_t k = 0; k < n_measurements; ++k)
+        for (const auto& g : c.gates()) out[k].push_back((int)((idx[k] >> g.target()) & 1ull));
     return out;
 }
 
+// sum_k prod_g (1 - 2 bit_g(k)) |a_k|^2 in the rotated basis (computer.cc:
+// 275-315) = <psi'| Z(mask) |psi'>, mask the XOR of the gate targets.
 double Computer::perfect_measure_circuit(const Circuit& c) {
-    std::vector<cplx> a = basis_rotated_probs(c);
-    double sum = 0.0;
-    for (uint64_t k = 0; k < a.size(); ++k) {
-        double v = 1.0;
-        for (const auto& g : c.gates()) v *= 1.0 - 2.0 * (double)((k >> g.target()) & 1ull);
-        sum += v * std::norm(a[k]);
-    }
-    return sum;
+    Computer tmp = basis_rotated(c);
+    qfe_pauli t{};
+    for (const auto& g : c.gates()) t.z ^= 1ull << g.target();
+    t.phase.re = 1.0;
+    t.coeff.re = 1.0;
+    qfe_c64 out{};
+    check(qfe_expval(tmp.s_, &t, 1, &out), "perfect_measure_circuit");
+    return out.re;
 }
 
 std::vector<cplx> Computer::get_coeff_vec() const {

@@ -293,7 +293,9 @@ class Computer {
     // Lower a circuit to an op stream (gates, and exponential segments as one op).
     void lower(const Circuit& c, std::vector<qfe_op>& ops) const;
     void apply_ops(const std::vector<qfe_op>& ops);
-    std::vector<cplx> basis_rotated_probs(const Circuit& c);
+    // copy of the state in the measurement basis of c (computer.cc:141-165)
+    Computer basis_rotated(const Circuit& c) const;
+    std::vector<uint64_t> sample(size_t n_measurements) const;
 };
 
 bool operator==(const Computer& a, const Computer& b);  // computer.cc:968-973
