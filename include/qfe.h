@@ -203,6 +203,12 @@ int qfe_program_gradient(qfe_program p, const double* params, double* grad, doub
  * state U(t)|ref> (measure_gradient3, uccvqeabc.py:253-283).  Operators are
  * delimited by op_offsets; processed in batches of up to `batch` operators
  * per launch. */
+/* PQE residual amplitudes (UCCNPQE.get_residual_vector, uccnpqe.py:249-292:
+ * apply_circuit(U), apply_operator(H), apply_circuit(U.adjoint()), then
+ * coeff at each excited determinant): out[i] = <idx[i]| U(t)^dag H U(t) |ref>,
+ * idx[i] a basis index of the register.  The caller multiplies by the
+ * determinant's phase. */
+int qfe_program_residuals(qfe_program p, const double* params, const uint64_t* idx, uint64_t count, qfe_c64* out);
 int qfe_program_pool_gradients(qfe_program p, const double* params, const qfe_pauli* pool_terms,
                                const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 
@@ -247,6 +253,10 @@ int qfe_template_gradient(qfe_template t, const double* params, const double* in
 /* ADAPT pool gradients 2 Re<H psi|A_k psi> (uccvqeabc.py:253-283) for nops
  * anti-Hermitian operators given as Pauli sums (pool_terms split by
  * op_offsets); `batch` operators per launch (0: 64). */
+/* PQE residual amplitudes on the sector (as qfe_program_residuals:
+ * uccnpqe.py:249-292); every idx[i] must lie in the template's sector. */
+int qfe_template_residuals(qfe_template t, const double* params, const double* inputs, const uint64_t* idx,
+                           uint64_t count, qfe_c64* out);
 int qfe_template_pool_gradients(qfe_template t, const double* params, const double* inputs, const qfe_pauli* pool_terms,
                                 const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 

@@ -98,6 +98,14 @@ class UCCProgram {
         py::gil_scoped_release rel;
         check(qfe_program_prepare(p_, t.data(), out.handle()), "UCCProgram.prepare");
     }
+    std::vector<cplx> residuals(const std::vector<double>& t, const std::vector<uint64_t>& idx) {
+        need(t);
+        std::vector<cplx> out(idx.size());
+        py::gil_scoped_release rel;
+        check(qfe_program_residuals(p_, t.data(), idx.data(), idx.size(), reinterpret_cast<qfe_c64*>(out.data())),
+              "UCCProgram.residuals");
+        return out;
+    }
     std::vector<double> pool_gradients(const std::vector<double>& t, const std::vector<QubitOperator>& ops,
                                        size_t batch) {
         need(t);
@@ -185,6 +193,16 @@ class SectorTemplate {
         py::gil_scoped_release rel;
         check(qfe_template_gradient(t_, t.data(), inputs.data(), g.data(), &e), "SectorTemplate.gradient");
         return {e, g};
+    }
+    std::vector<cplx> residuals(const std::vector<double>& t, const std::vector<uint64_t>& idx,
+                                const std::vector<double>& inputs) {
+        need(t, inputs);
+        std::vector<cplx> out(idx.size());
+        py::gil_scoped_release rel;
+        check(qfe_template_residuals(t_, t.data(), inputs.data(), idx.data(), idx.size(),
+                                     reinterpret_cast<qfe_c64*>(out.data())),
+              "SectorTemplate.residuals");
+        return out;
     }
     std::vector<double> pool_gradients(const std::vector<double>& t, const std::vector<QubitOperator>& ops, size_t batch,
                                        const std::vector<double>& inputs) {
@@ -597,6 +615,7 @@ PYBIND11_MODULE(_core, m) {
         .def("energy_graph", &UCCProgram::energy_graph)
         .def("gradient", &UCCProgram::gradient)
         .def("prepare", &UCCProgram::prepare)
+        .def("residuals", &UCCProgram::residuals, "amplitudes"_a, "indices"_a)
         .def("pool_gradients", &UCCProgram::pool_gradients, "amplitudes"_a, "operators"_a, "batch"_a = 64)
         .def_property_readonly("nqubit", &UCCProgram::nqubit)
         .def_property_readonly("nparam", &UCCProgram::nparam);
@@ -627,6 +646,8 @@ PYBIND11_MODULE(_core, m) {
              "nqubit"_a, "ref_index"_a, "generators"_a, "param_index"_a, "scale"_a, "nparam"_a, "hamiltonian_map"_a)
         .def("energy", &SectorTemplate::energy, "amplitudes"_a, "inputs"_a = std::vector<double>())
         .def("prepare", &SectorTemplate::prepare)
+        .def("residuals", &SectorTemplate::residuals, "amplitudes"_a, "indices"_a,
+             "inputs"_a = std::vector<double>())
         .def("gradient", &SectorTemplate::gradient, "amplitudes"_a, "inputs"_a = std::vector<double>())
         .def("pool_gradients", &SectorTemplate::pool_gradients, "amplitudes"_a, "operators"_a, "batch"_a = 64,
              "inputs"_a = std::vector<double>())

@@ -504,6 +504,32 @@ int qfe_program_gradient(qfe_program p, const double* params, double* grad, doub
     return QFE_OK;
 }
 
+int qfe_program_residuals(qfe_program p, const double* params, const uint64_t* idx, uint64_t count, qfe_c64* out) {
+    if (!p || (count && (!idx || !out)) || (p->nparam && !params))
+        return fail(QFE_ERR_ARG, "qfe_program_residuals: invalid argument");
+    for (uint64_t i = 0; i < count; ++i)
+        if (idx[i] >> p->n) return fail(QFE_ERR_ARG, "qfe_program_residuals: basis index out of range");
+    int rc = ensure_states(p, true);
+    if (rc) return rc;
+    rc = prepare(p, params, p->psi);
+    if (rc) return rc;
+    // sigma = H U|ref>, then U^dag: generators last to first, terms reversed
+    rc = launch_apply_sum_async(p->psi->d, p->sig->d, p->n, p->ham.g, p->ham.ngroups, p->ham.z, p->ham.c);
+    if (rc) return rc;
+    std::vector<HostOp> ops;
+    for (uint64_t kk = p->gen_off.size() - 1; kk-- > 0;) {
+        rc = lower_generator(p, kk, params, -1.0, ops);
+        if (rc) return rc;
+    }
+    rc = run_ops(p->sig, ops);
+    if (rc) return rc;
+    for (uint64_t i = 0; i < count; ++i) {
+        rc = qfe_state_get_amp(p->sig, idx[i], &out[i]);
+        if (rc) return rc;
+    }
+    return QFE_OK;
+}
+
 int qfe_program_pool_gradients(qfe_program p, const double* params, const qfe_pauli* pool_terms,
                                const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads) {
     if (!p || (nops && (!pool_terms || !op_offsets || !grads)) || (p->nparam && !params))

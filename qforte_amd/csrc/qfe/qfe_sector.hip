@@ -767,6 +767,13 @@ __global__ __launch_bounds__(kWideTPB) void k_sec_ref(const SlotDesc* __restrict
     if (r < sd.t->D) sd.psi[r] = make_double2(r == sd.t->ref_rank ? 1.0 : 0.0, 0.0);
 }
 
+// out[i] = sigma[rank[i]] (PQE residual amplitudes).
+__global__ void k_sec_gather(const SlotDesc* __restrict__ slots, const uint64_t* __restrict__ rank, uint64_t count,
+                             double2* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = slots[0].sig[rank[i]];
+}
+
 // Scatter a sector state into a zeroed full state.
 __global__ void k_sector_scatter(const double2* __restrict__ psi, uint64_t D, uint32_t Db, const uint64_t* __restrict__ afull,
                                  const uint64_t* __restrict__ bfull, double2* __restrict__ out) {
@@ -1917,6 +1924,44 @@ int qfe_template_gradient(qfe_template t, const double* params, const double* in
     if (rc) return rc;
     gradient_from_overlaps(t, acc.data(), grad);
     if (energy) *energy = acc[t->ngen];
+    return QFE_OK;
+}
+
+int qfe_template_residuals(qfe_template t, const double* params, const double* inputs, const uint64_t* idx,
+                           uint64_t count, qfe_c64* out) {
+    if (!t || (count && (!idx || !out)) || (t->nparam && !params) || (t->ninputs && !inputs))
+        return fail(QFE_ERR_ARG, "qfe_template_residuals: null argument");
+    std::vector<uint64_t> rank(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        if (idx[i] >> t->n) return fail(QFE_ERR_ARG, "qfe_template_residuals: basis index out of range");
+        const int32_t ra = t->sp.rank[0][compress(idx[i], 0, t->norb)];
+        const int32_t rb = t->sp.rank[1][compress(idx[i], 1, t->norb)];
+        if (ra < 0 || rb < 0) return fail(QFE_ERR_ARG, "qfe_template_residuals: basis index outside the sector");
+        rank[i] = (uint64_t)ra * t->Db + (uint64_t)rb;
+    }
+    // sigma = H U|ref>, then U^dag on sigma: every step inverted, last step
+    // first (psi is carried back to |ref> alongside)
+    Single S;
+    int rc = single_psi_sigma(t, params, inputs, S);
+    if (rc) return rc;
+    hipStream_t st = ctx().stream;
+    for (uint32_t sdx = (uint32_t)t->step_pairs.size(); sdx-- > 0;) {
+        if (!t->step_pairs[sdx]) continue;
+        hipLaunchKernelGGL(k_sec_step, dim3(wide_blocks(t->step_pairs[sdx])), dim3(kWideTPB), 0, st, S.ddesc, sdx, 1, 1);
+        QFE_HIP(hipGetLastError());
+    }
+    if (!count) return QFE_OK;
+    void* scratch = nullptr;
+    const size_t o_out = align256(count * 8);
+    rc = scratch_reserve(o_out + count * sizeof(double2), &scratch);
+    if (rc) return rc;
+    QFE_HIP(hipMemcpyAsync(scratch, rank.data(), count * 8, hipMemcpyHostToDevice, st));
+    double2* dres = (double2*)((char*)scratch + o_out);
+    hipLaunchKernelGGL(k_sec_gather, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, S.ddesc,
+                       (const uint64_t*)scratch, count, dres);
+    QFE_HIP(hipGetLastError());
+    QFE_HIP(hipMemcpyAsync(out, dres, count * sizeof(double2), hipMemcpyDeviceToHost, st));
+    QFE_HIP(hipStreamSynchronize(st));
     return QFE_OK;
 }
 

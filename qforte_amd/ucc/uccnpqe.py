@@ -90,15 +90,20 @@ class UCCNPQE(UCC, AnsatzAlgorithm):
         """U(t)|ref>, apply H, un-apply U, read the excited amplitudes (uccnpqe.py:249-292)."""
         if self._pool_type == "sa_SD":
             raise ValueError("Must use single term particle-hole nbody operators for residual calculation")
-        U = self.ansatz_circuit(trial_amps)
-        qc = _core.Computer(self._nqb)
-        qc.apply_circuit(self._Uprep)
-        qc.apply_circuit(U)
-        qc.apply_operator(self._qb_ham)
-        qc.apply_circuit(U.adjoint())
+        if self._can_use_program():
+            # U^dag H U |ref> on the compiled program, read at the excited determinants
+            amps = self._get_program().residuals([float(x) for x in trial_amps], [I for I, _ in self._excited_dets])
+        else:
+            U = self.ansatz_circuit(trial_amps)
+            qc = _core.Computer(self._nqb)
+            qc.apply_circuit(self._Uprep)
+            qc.apply_circuit(U)
+            qc.apply_operator(self._qb_ham)
+            qc.apply_circuit(U.adjoint())
+            amps = [qc.coeff(_core.QubitBasis(I)) for I, _ in self._excited_dets]
         residuals = []
-        for I, phase in self._excited_dets:
-            r = qc.coeff(_core.QubitBasis(I)) * phase
+        for a, (I, phase) in zip(amps, self._excited_dets):
+            r = a * phase
             if abs(np.imag(r)) > 1.0e-14:
                 raise ValueError("residual has imaginary component, something went wrong!!")
             residuals.append(float(np.real(r)))

@@ -128,9 +128,9 @@ def h4(qf):
 @pytest.mark.parametrize("qubit", [False, True])
 def test_compact_excitations_match_standard_circuits(qf, qubit):
     # tests/test_feb_qeb_circuits.py:6-62 on H4/STO-6G: the same energies with
-    # fewer CNOTs.  The VQE drivers run both forms through the compiled
-    # program (each compact circuit is exp(t kappa)); PQE residuals apply the
-    # compact gate circuits themselves.
+    # fewer CNOTs.  The drivers run both forms through the compiled program
+    # (each compact circuit is exp(t kappa)); the gate circuits themselves are
+    # exercised below with use_program=False.
     mol = h4(qf)
     cases = [(qf.UCCNVQE, {"pool_type": "SD"}), (qf.UCCNVQE, {"pool_type": "sa_SD"}),
              (qf.ADAPTVQE, {"pool_type": "SD", "avqe_thresh": 1.0e-3}), (qf.UCCNPQE, {"pool_type": "SD"})]
@@ -183,3 +183,30 @@ def test_compact_pqe_jacobi_and_bfgs_agree(qf):
     bfgs = qf.UCCNPQE(mol, compact_excitations=True, qubit_excitations=False)
     bfgs.run(optimizer="BFGS", pool_type="SD")
     assert jacobi.get_gs_energy() == approx(bfgs.get_gs_energy(), abs=1.0e-8)
+
+
+def test_pqe_residuals_sector_full_space_and_gate_path(qf):
+    # r_mu = <Phi_mu| U^dag H U |ref> phase_mu (uccnpqe.py:249-292) on the
+    # sector template, on the full-space program and through Computer with the
+    # standard and the compact circuits
+    mol = h4(qf)
+    t = None
+    results = []
+    for compact, use_program, use_sector in [(False, True, True), (False, True, False), (False, False, True),
+                                             (True, False, True)]:
+        alg = qf.UCCNPQE(mol, compact_excitations=compact, use_program=use_program, use_sector=use_sector)
+        alg._pool_type = "SD"
+        alg.fill_pool()
+        alg._tops = list(range(len(alg._pool_obj)))
+        if t is None:
+            t = [0.07 * ((3 * k) % 7 - 3) for k in range(len(alg._tops))]
+        alg._tamps = list(t)
+        alg._excited_dets, alg._res_vec_evals, alg._res_m_evals = [], 0, 0
+        alg.fill_excited_dets()
+        results.append(np.asarray(alg.get_residual_vector(t)))
+        if use_program:
+            kind = "SectorTemplate" if use_sector else "UCCProgram"
+            assert type(alg._get_program()).__name__ == kind
+    assert np.abs(results[0]).max() > 1e-3
+    for r in results[1:]:
+        assert np.abs(r - results[0]).max() < 1e-12

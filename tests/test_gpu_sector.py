@@ -269,3 +269,12 @@ def test_large_sector_grid_wide_path_matches_program(qf):
     # repeated calls give the same numbers (fixed-order sums)
     e2, gr2 = tpl.gradient(t, inputs)
     assert e2 == e and list(gr2) == list(gr)
+    # PQE residual amplitudes <I| U^dag H U |ref> at sector determinants:
+    # the reference and the largest amplitudes of U|ref>
+    amps = np.asarray(a.get_coeff_vec())
+    sel = [int(ridx)] + [int(i) for i in np.argsort(-np.abs(amps))[:40]]
+    r_t = np.asarray(tpl.residuals(t, sel, inputs))
+    r_p = np.asarray(prog.residuals(t, sel))
+    assert np.abs(r_p).max() > 1e-3
+    assert np.max(np.abs(r_t - r_p)) < 1e-11
+    assert abs(r_p[0].real - e_p) < E_TOL  # <ref|U^dag H U|ref> is the energy
