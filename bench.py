@@ -482,7 +482,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the state-vector path is single-GPU; see DESIGN.md for how it would shard).", file=sys.stderr)
+              "(the state-vector path runs on one GPU).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -383,11 +383,61 @@ __device__ __forceinline__ void group_axis(double2 (&a)[8], const uint32_t (&e)[
     }
 }
 
+// Logical -> physical index map of an LDS tile: P(l) = M l ^ off over GF(2),
+// M given by its columns col[b] = M e_b.  CNOT, anti-controlled CNOT and X
+// gates on tile bits permute amplitudes, so instead of moving data they
+// update the map (P' = P o G: a CNOT c -> t adds col[t] to col[c], an X on t
+// adds col[t] to off); every other op reads and writes tile[P(l)] for its
+// logical indices l, and the write-back reads tile[P(l)] for amplitude l.
+template <int KQ>
+struct TileMap {
+    uint32_t col[KQ];
+    uint32_t off;
+    bool perm;  // false: P is the identity (no translation)
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int b = 0; b < KQ; ++b) col[b] = 1u << b;
+        off = 0;
+        perm = false;
+    }
+    // M v (linear part only)
+    __device__ __forceinline__ uint32_t lin(uint32_t v) const {
+        if (!perm) return v;
+        uint32_t r = 0;
+#pragma unroll
+        for (int b = 0; b < KQ; ++b) r ^= ((v >> b) & 1u) ? col[b] : 0u;
+        return r;
+    }
+    __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return perm ? (lin(l) ^ off) : l; }
+    // column of tile bit t (t uniform): a compare chain keeps col[] in registers
+    __device__ __forceinline__ uint32_t column(uint32_t t) const {
+        uint32_t r = 0;
+#pragma unroll
+        for (int b = 0; b < KQ; ++b) r = ((uint32_t)b == t) ? col[b] : r;
+        return r;
+    }
+    // G = X on t, controlled by tile bit c when cmask != 0 (cval: required value)
+    __device__ __forceinline__ void apply_swap(uint32_t t, uint32_t cmask, uint32_t cval) {
+        const uint32_t ct = column(t);
+        if (cmask) {
+            const uint32_t c = (uint32_t)__builtin_ctz(cmask);
+#pragma unroll
+            for (int b = 0; b < KQ; ++b)
+                if ((uint32_t)b == c) col[b] ^= ct;
+            if (!(cval & cmask)) off ^= ct;
+        } else {
+            off ^= ct;
+        }
+        perm = true;
+    }
+};
+
 // 2 or 3 consecutive U1 ops with distinct tile targets t0, t1 (, t2): every
 // thread holds the 8 amplitudes spanned by the three bits (a free bit pads a
 // pair) and applies the ops in order, partners at compile-time positions.
-template <uint32_t T>
-__device__ __forceinline__ void fused_group(double2* tile, uint64_t base, const DevOp* __restrict__ g, uint32_t cnt) {
+template <uint32_t T, int KQ>
+__device__ __forceinline__ void fused_group(double2* tile, const TileMap<KQ>& P, uint64_t base,
+                                            const DevOp* __restrict__ g, uint32_t cnt) {
     const uint32_t t0 = g[0].t, t1 = g[1].t;
     uint32_t t2 = 0;
     if (cnt > 2) {
@@ -399,20 +449,23 @@ __device__ __forceinline__ void fused_group(double2* tile, uint64_t base, const 
     const uint32_t lo = min(t0, min(t1, t2)), hi = max(t0, max(t1, t2));
     const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
     const uint32_t b0 = 1u << t0, b1 = 1u << t1, b2 = 1u << t2;
+    const uint32_t p0 = P.lin(b0), p1 = P.lin(b1), p2 = P.lin(b2);
     for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
         const uint32_t eb = ins0_32(ins0_32(ins0_32(it, lo), mid), hi);
-        uint32_t e[8];
+        const uint32_t pb = P(eb);
+        uint32_t e[8], pe[8];
         double2 a[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             e[i] = eb | ((i & 1) ? b0 : 0u) | ((i & 2) ? b1 : 0u) | ((i & 4) ? b2 : 0u);
-            a[i] = tile[e[i]];
+            pe[i] = pb ^ ((i & 1) ? p0 : 0u) ^ ((i & 2) ? p1 : 0u) ^ ((i & 4) ? p2 : 0u);
+            a[i] = tile[pe[i]];
         }
         group_axis<0>(a, e, g[0], base);
         group_axis<1>(a, e, g[1], base);
         if (cnt > 2) group_axis<2>(a, e, g[2], base);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) tile[e[i]] = a[i];
+        for (int i = 0; i < 8; ++i) tile[pe[i]] = a[i];
     }
 }
 
@@ -450,6 +503,8 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
         const double2* src = s + (base | hioff[l >> nlow] | (l & lowmask));
         tile[l] = nt ? ld_nt(src) : *src;
     }
+    TileMap<KQ> P;
+    P.init();
     for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
         const DevOp& o = ops[oi];
@@ -457,21 +512,27 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             // a group of o.pad one-qubit ops on distinct tile bits: one LDS
             // round trip for the whole group
             const uint32_t cnt = o.pad;
-            fused_group<T>(tile, base, ops + oi, cnt);
+            fused_group<T, KQ>(tile, P, base, ops + oi, cnt);
             oi += cnt - 1;
             continue;
         }
         if ((base & o.omask) != o.oval) continue;  // uniform across the block
         switch (o.kind) {
             case K_U1: {
-                const uint32_t tb = 1u << o.t;
-                if (o.hb) {  // X on the target: swap the pair, no arithmetic
+                const uint32_t tb = 1u << o.t, ptb = P.lin(tb);
+                if (o.hb) {  // X on the target
+                    if (__popc((uint32_t)o.cmask) <= 1) {
+                        // a permutation of the tile: the map absorbs it
+                        P.apply_swap(o.t, (uint32_t)o.cmask, (uint32_t)o.cval);
+                        break;
+                    }
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
                         const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
-                        const double2 a = tile[l0];
-                        tile[l0] = tile[l0 | tb];
-                        tile[l0 | tb] = a;
+                        const uint32_t q0 = P(l0);
+                        const double2 a = tile[q0];
+                        tile[q0] = tile[q0 ^ ptb];
+                        tile[q0 ^ ptb] = a;
                     }
                     break;
                 }
@@ -479,56 +540,59 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
                 // the pairs of one op are disjoint: all of a thread's pairs are
                 // read before any is written, so the LDS reads overlap
                 constexpr uint32_t PP = (T / 2 + kFusedTPB - 1) / kFusedTPB;
-                uint32_t l0[PP];
+                uint32_t q0[PP];
                 bool on[PP];
                 double2 a[PP], b[PP];
 #pragma unroll
                 for (uint32_t r = 0; r < PP; ++r) {
                     const uint32_t p = threadIdx.x + r * kFusedTPB;
-                    l0[r] = ins0_32(p, o.t);
-                    on[r] = p < T / 2 && (l0[r] & (uint32_t)o.cmask) == (uint32_t)o.cval;
+                    const uint32_t l0 = ins0_32(p, o.t);
+                    q0[r] = P(l0);
+                    on[r] = p < T / 2 && (l0 & (uint32_t)o.cmask) == (uint32_t)o.cval;
                     if (on[r]) {
-                        a[r] = tile[l0[r]];
-                        b[r] = tile[l0[r] | tb];
+                        a[r] = tile[q0[r]];
+                        b[r] = tile[q0[r] ^ ptb];
                     }
                 }
 #pragma unroll
                 for (uint32_t r = 0; r < PP; ++r) {
                     if (on[r]) {
                         u2x2(a[r], b[r], m);
-                        tile[l0[r]] = a[r];
-                        tile[l0[r] | tb] = b[r];
+                        tile[q0[r]] = a[r];
+                        tile[q0[r] ^ ptb] = b[r];
                     }
                 }
                 break;
             }
             case K_DIAG1: {
                 if (o.tin) {
-                    const uint32_t tb = 1u << o.t;
+                    const uint32_t ptb = P.lin(1u << o.t);
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
                         const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
-                        tile[l0] = cmul(o.m[0], tile[l0]);
-                        tile[l0 | tb] = cmul(o.m[1], tile[l0 | tb]);
+                        const uint32_t q0 = P(l0);
+                        tile[q0] = cmul(o.m[0], tile[q0]);
+                        tile[q0 ^ ptb] = cmul(o.m[1], tile[q0 ^ ptb]);
                     }
                 } else {
                     const double2 f = ((base >> o.t) & 1ull) ? o.m[1] : o.m[0];
                     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
                         if ((l & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
-                        tile[l] = cmul(f, tile[l]);
+                        const uint32_t q = P(l);
+                        tile[q] = cmul(f, tile[q]);
                     }
                 }
                 break;
             }
             case K_U2: {
                 const uint32_t lo = o.t < o.c ? o.t : o.c, hi = o.t < o.c ? o.c : o.t;
-                const uint32_t tb = 1u << o.t, cb = 1u << o.c;
+                const uint32_t ptb = P.lin(1u << o.t), pcb = P.lin(1u << o.c);
                 for (uint32_t p = threadIdx.x; p < T / 4; p += kFusedTPB) {
-                    const uint32_t j = ins0_32(ins0_32(p, lo), hi);
-                    const uint32_t idx[4] = {j, j | tb, j | cb, j | cb | tb};
+                    const uint32_t q = P(ins0_32(ins0_32(p, lo), hi));
+                    const uint32_t idx[4] = {q, q ^ ptb, q ^ pcb, q ^ pcb ^ ptb};
                     double2 v[4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) v[q] = tile[idx[q]];
+                    for (int k = 0; k < 4; ++k) v[k] = tile[idx[k]];
 #pragma unroll
                     for (int row = 0; row < 4; ++row) {
                         double2 acc = make_double2(0.0, 0.0);
@@ -549,19 +613,21 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
                     const double2 fm = make_double2(c + sn * ph.y, -sn * ph.x);
                     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
                         const bool neg = ((__popc(l & zl) & 1) != 0) != bneg;
-                        tile[l] = cmul(neg ? fm : fp, tile[l]);
+                        const uint32_t q = P(l);
+                        tile[q] = cmul(neg ? fm : fp, tile[q]);
                     }
                 } else {
-                    const uint32_t xl = (uint32_t)o.x;
+                    const uint32_t xl = (uint32_t)o.x, pxl = P.lin(xl);
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
                         const uint32_t la = ins0_32(p, o.hb);
                         const uint32_t lb = la ^ xl;
-                        double2 a = tile[la], b = tile[lb];
+                        const uint32_t qa = P(la), qb = qa ^ pxl;
+                        double2 a = tile[qa], b = tile[qb];
                         const bool na = ((__popc(la & zl) & 1) != 0) != bneg;
                         const bool nb = ((__popc(lb & zl) & 1) != 0) != bneg;
                         pexp_pair(a, b, c, sn, ph, na, nb);
-                        tile[la] = a;
-                        tile[lb] = b;
+                        tile[qa] = a;
+                        tile[qb] = b;
                     }
                 }
                 break;
@@ -573,10 +639,11 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
     __syncthreads();
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
         double2* dst = s + (base | hioff[l >> nlow] | (l & lowmask));
+        const double2 v = tile[P(l)];
         if (nt)
-            st_nt(dst, tile[l]);
+            st_nt(dst, v);
         else
-            *dst = tile[l];
+            *dst = v;
     }
 }
 
