@@ -208,6 +208,13 @@ __device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, 
     return s2 * (sa.x * k1.x + sa.y * k1.y) + s1 * (sb.x * k2.x + sb.y * k2.y);
 }
 
+// v with its sign bit flipped when neg = 1u << 31: adding flip_sign(c, neg) is
+// bit-identical to fma(+-1.0, c, acc), with one integer op for the sign.
+// Determinant indices and Z masks fit 32 bits (at most 32 qubits).
+__device__ __forceinline__ double flip_sign(double v, uint32_t neg) {
+    return __hiloint2double(__double2hiint(v) ^ (int)neg, __double2loint(v));
+}
+
 // Rotate every pair of entry `en` by (c, sgn * s); pairs of one step are
 // disjoint, so kUnroll of them are in flight per thread (loads first).
 template <int THREADS>
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(THREADS) void k_sec_ansatz(const SlotDesc* __restri
 // ---- 3. energy work items: one workgroup per (request, item) ---------------------------
 __global__ __launch_bounds__(kItemThreads) void k_sec_energy(const SlotDesc* __restrict__ slots,
                                                              const uint32_t* __restrict__ prefix, uint32_t nslots) {
-    __shared__ uint64_t tz_s[kTermLds];
+    __shared__ uint32_t tz_s[kTermLds];
     __shared__ double2 c_s[kTermLds];
     __shared__ double2 red[16];
     const uint32_t slot = find_slot(prefix, nslots, blockIdx.x);
@@ -370,17 +377,17 @@ __global__ __launch_bounds__(kItemThreads) void k_sec_energy(const SlotDesc* __r
         if (lds) {
             __syncthreads();
             for (uint32_t k = tid; k < nt; k += kItemThreads) {
-                tz_s[k] = T->tz[t0 + k];
+                tz_s[k] = (uint32_t)T->tz[t0 + k];
                 c_s[k] = coef[t0 + k];
             }
             __syncthreads();
         }
         const uint32_t lb = gr.lb_len;
         const uint32_t p0 = whole ? 0u : it.p0, p1 = whole ? gr.la_len * lb : it.p1;
-        const uint64_t x = gr.x;
+        const uint32_t x = (uint32_t)gr.x;
         double gacc = 0.0;
         for (uint32_t i0 = p0 + tid; i0 < p1; i0 += kUnroll * kItemThreads) {
-            uint64_t j2[kUnroll];
+            uint32_t j2[kUnroll];
             double2 v[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(kItemThreads) void k_sec_energy(const SlotDesc* __r
                     divmod_u(i, lb, gr.inv_lb, ia, ib);
                     const int2 pa = T->lists[gr.la_off + ia];
                     const int2 pb = T->lists[gr.lb_off + ib];
-                    j2[u] = (T->afull[pa.x] | T->bfull[pb.x]) ^ x;
+                    j2[u] = (uint32_t)(T->afull[pa.x] | T->bfull[pb.x]) ^ x;
                     v[u] = cconjmul(psi[(uint64_t)pa.x * Db + (uint64_t)pb.x], psi[(uint64_t)pa.y * Db + (uint64_t)pb.y]);
                 }
             }
@@ -400,13 +407,13 @@ __global__ __launch_bounds__(kItemThreads) void k_sec_energy(const SlotDesc* __r
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) tr[u] = ti[u] = 0.0;
             for (uint32_t t = 0; t < nt; ++t) {
-                const uint64_t z = lds ? tz_s[t] : T->tz[t0 + t];
+                const uint32_t z = lds ? tz_s[t] : (uint32_t)T->tz[t0 + t];
                 const double2 c = lds ? c_s[t] : coef[t0 + t];
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u) {
-                    const double sg = parity_sign(j2[u] & z);
-                    tr[u] += sg * c.x;
-                    ti[u] += sg * c.y;
+                    const uint32_t neg = (uint32_t)__popc(j2[u] & z) << 31;
+                    tr[u] += flip_sign(c.x, neg);
+                    ti[u] += flip_sign(c.y, neg);
                 }
             }
 #pragma unroll
@@ -442,14 +449,14 @@ __global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ s
     constexpr int kHpsiPer = PER;
     constexpr uint32_t kHpsiSpan = 256 * PER;  // determinants per block
     __shared__ uint4 hdr[kHpsiGroups];  // xa, xb, first term, term count
-    __shared__ uint64_t gx[kHpsiGroups];
-    __shared__ uint64_t tzs[kHpsiTerms];
+    __shared__ uint32_t gx[kHpsiGroups];
+    __shared__ uint32_t tzs[kHpsiTerms];
     __shared__ double2 cfs[kHpsiTerms];
     const SlotDesc sd = slots[blockIdx.y];
     const SecTables* __restrict__ T = sd.t;
     if ((uint64_t)blockIdx.x * kHpsiSpan >= T->D) return;  // the whole block: a smaller request of the launch
     uint32_t sa[kHpsiPer], sb[kHpsiPer];
-    uint64_t j[kHpsiPer];
+    uint32_t j[kHpsiPer];
     bool live[kHpsiPer];
     double sr[kHpsiPer], si[kHpsiPer];
 #pragma unroll
@@ -459,7 +466,7 @@ __global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ s
         const uint32_t ra = live[q] ? (uint32_t)(r / T->Db) : 0u, rb = live[q] ? (uint32_t)(r % T->Db) : 0u;
         sa[q] = T->astr[ra];
         sb[q] = T->bstr[rb];
-        j[q] = T->afull[ra] | T->bfull[rb];
+        j[q] = (uint32_t)(T->afull[ra] | T->bfull[rb]);
         sr[q] = si[q] = 0.0;
     }
     const double2* __restrict__ coef = sd.coef ? sd.coef : T->tcoef;
@@ -472,11 +479,11 @@ __global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ s
         for (uint32_t g = ch.x + threadIdx.x; g < ch.y; g += blockDim.x) {
             const SecGroup& gr = T->groups[g];
             hdr[g - ch.x] = make_uint4(gr.xa, gr.xb, gr.t_first, gr.t_count);
-            gx[g - ch.x] = gr.x;
+            gx[g - ch.x] = (uint32_t)gr.x;
         }
         if (tl)
             for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-                tzs[t - t0] = T->tz[t];
+                tzs[t - t0] = (uint32_t)T->tz[t];
                 cfs[t - t0] = coef[t];
             }
         __syncthreads();
@@ -491,7 +498,7 @@ __global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ s
                 any |= on[q];
             }
             if (!any) continue;
-            uint64_t j2[kHpsiPer];
+            uint32_t j2[kHpsiPer];
             double tr[kHpsiPer], ti[kHpsiPer];
 #pragma unroll
             for (int q = 0; q < kHpsiPer; ++q) {
@@ -499,13 +506,13 @@ __global__ __launch_bounds__(256) void k_sec_hpsi(const SlotDesc* __restrict__ s
                 tr[q] = ti[q] = 0.0;
             }
             for (uint32_t t = h.z; t < h.z + h.w; ++t) {
-                const uint64_t z = tl ? tzs[t - t0] : T->tz[t];
+                const uint32_t z = tl ? tzs[t - t0] : (uint32_t)T->tz[t];
                 const double2 cf = tl ? cfs[t - t0] : coef[t];
 #pragma unroll
                 for (int q = 0; q < kHpsiPer; ++q) {
-                    const double sg = parity_sign(j2[q] & z);
-                    tr[q] += sg * cf.x;
-                    ti[q] += sg * cf.y;
+                    const uint32_t neg = (uint32_t)__popc(j2[q] & z) << 31;
+                    tr[q] += flip_sign(cf.x, neg);
+                    ti[q] += flip_sign(cf.y, neg);
                 }
             }
 #pragma unroll
