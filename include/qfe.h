@@ -90,6 +90,11 @@ int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass);
  * orbit over 2 or 4 lanes of a wave); no reference counterpart (autotuner hook). */
 int qfe_set_pass_tuning(int orbit_max_dims);
 int qfe_get_pass_tuning(int* orbit_max_dims);
+/* Largest number of determinant pairs for which a sector template stores one
+ * 8-byte pair word per pair (templates above it rotate from the compact
+ * string lists); applies to templates created afterwards.  Default 2^25.  No
+ * reference counterpart (memory / speed trade-off of the sector engine). */
+int qfe_set_sector_word_limit(uint64_t max_pair_words);
 /* HIP events on the library stream, for timing kernels from the host
  * (no reference counterpart). */
 int qfe_event_create(void** event);

@@ -45,6 +45,7 @@ from ._core import (  # noqa: F401
     prepare_computer_from_gates,
     reset_counters,
     set_pass_tuning,
+    set_sector_word_limit,
     set_tuning,
     stream_handle,
     synchronize,

@@ -675,6 +675,8 @@ PYBIND11_MODULE(_core, m) {
     m.def("set_tuning", [](int k, int l, int ops) { check(qfe_set_tuning(k, l, ops), "set_tuning"); }, "tile_qubits"_a = 0,
           "low_qubits"_a = 0, "max_ops_per_pass"_a = 0);
     m.def("set_pass_tuning", [](int o) { check(qfe_set_pass_tuning(o), "set_pass_tuning"); }, "orbit_max_dims"_a);
+    m.def("set_sector_word_limit", [](uint64_t n) { check(qfe_set_sector_word_limit(n), "set_sector_word_limit"); },
+          "max_pair_words"_a);
     m.def("get_pass_tuning", []() {
         int o = 0;
         qfe_get_pass_tuning(&o);

@@ -792,7 +792,6 @@ __global__ void k_sector_scatter(const double2* __restrict__ psi, uint64_t D, ui
 // Ansatz launch shapes by size tier: 0 = state in LDS, 256 threads; 1 = state
 // in LDS, 512 threads; 2 = state in device memory, 1024 threads.
 constexpr size_t kTier0Lds = 40 << 10;
-constexpr uint64_t kMaxPairWords = 1ull << 25;  // 256 MB of pair words per template at most
 constexpr int kTier2Threads = 1024;
 
 int launch_ansatz(int tier, unsigned nblocks, size_t lds, hipStream_t st, const SlotDesc* d) {
@@ -1216,7 +1215,7 @@ int template_build(qfe_template t, int n, uint64_t ref, const qfe_pauli* gen_ter
     {
         uint64_t total = 0;
         for (const auto& en : entries) total += (uint64_t)en.la_len * en.lb_len;
-        if (total <= kMaxPairWords) {
+        if (total <= ctx().tuning.sector_words) {
             words.reserve(total);
             for (auto& en : entries) {
                 en.w_off = (uint32_t)words.size();

@@ -380,6 +380,11 @@ int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass) {
     return QFE_OK;
 }
 
+int qfe_set_sector_word_limit(uint64_t max_pair_words) {
+    g_ctx.tuning.sector_words = max_pair_words;
+    return QFE_OK;
+}
+
 int qfe_set_pass_tuning(int orbit_max_dims) {
     if (orbit_max_dims < 1 || orbit_max_dims > 6) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: orbit_max_dims must be in [1, 6]");
     g_ctx.tuning.orbit_dims = orbit_max_dims;

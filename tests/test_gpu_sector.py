@@ -278,3 +278,35 @@ def test_large_sector_grid_wide_path_matches_program(qf):
     assert np.abs(r_p).max() > 1e-3
     assert np.max(np.abs(r_t - r_p)) < 1e-11
     assert abs(r_p[0].real - e_p) < E_TOL  # <ref|U^dag H U|ref> is the energy
+
+
+def test_pair_words_and_list_form_agree_bit_for_bit(qf):
+    """Templates above the pair-word limit rotate from the string lists; both
+    forms apply the same arithmetic to every pair, so energies, gradients
+    and served results are identical (tiers 0, 1 and 2: 8, 16 and 18 qubits)."""
+    rng = np.random.default_rng(17)
+    for norb, na, nb in [(4, 2, 2), (8, 4, 4), (9, 5, 4)]:
+        _, _, ref = synthetic_case(qf, norb, na, nb, 0)
+        pool = sd_pool(qf, ref)
+        hmap = qf.HamiltonianMap(norb)
+        tops = list(range(0, len(pool), 3))
+        qf.set_sector_word_limit(0)
+        try:
+            listed = qf.compile_ucc_template(2 * norb, ref, hmap, pool, tops)
+        finally:
+            qf.set_sector_word_limit(1 << 25)
+        worded = qf.compile_ucc_template(2 * norb, ref, hmap, pool, tops)
+        h, g = qf.synthetic_integrals(norb, 50 + norb)
+        inputs = hmap.pack(0.0, h, g)
+        t = list(rng.normal(0.0, 0.1, len(tops)))
+        assert listed.energy(t, inputs) == worded.energy(t, inputs), norb
+        e1, g1 = listed.gradient(t, inputs)
+        e2, g2 = worded.gradient(t, inputs)
+        assert e1 == e2 and list(g1) == list(g2), norb
+        eng = qf.ServingEngine(hbm_budget_bytes=1 << 30, max_active=64)
+        ids = [eng.submit(tpl, t, inputs) for tpl in (listed, worded, listed, worded)]
+        gids = [eng.submit(tpl, t, inputs, gradient=True) for tpl in (listed, worded)]
+        eng.drain()
+        vals = [eng.results[i] for i in ids]
+        assert vals[0] == vals[1] == vals[2] == vals[3], norb
+        assert list(eng.gradients[gids[0]]) == list(eng.gradients[gids[1]]), norb
