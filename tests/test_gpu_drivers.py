@@ -210,3 +210,17 @@ def test_pqe_residuals_sector_full_space_and_gate_path(qf):
     assert np.abs(results[0]).max() > 1e-3
     for r in results[1:]:
         assert np.abs(r - results[0]).max() < 1e-12
+
+
+def test_compact_adapt_and_pqe_jacobi_match_bfgs(qf):
+    # tests/test_adapt_vqe.py:42-90 (compact qubit excitations, GSD pool) and
+    # tests/test_uccsd.py:86-122 (compact fermionic PQE): Jacobi with DIIS and
+    # BFGS reach the same energy
+    mol = h4(qf)
+    jacobi = qf.ADAPTVQE(mol, compact_excitations=True, qubit_excitations=True, diis_max_dim=8,
+                         print_summary_file=False)
+    jacobi.run(optimizer="jacobi", pool_type="GSD", avqe_thresh=0.001)
+    bfgs = qf.ADAPTVQE(mol, compact_excitations=True, qubit_excitations=True, print_summary_file=False)
+    bfgs.run(optimizer="BFGS", pool_type="GSD", avqe_thresh=0.001)
+    assert jacobi.get_gs_energy() == approx(bfgs.get_gs_energy(), abs=1.0e-8)
+    assert jacobi._n_cnot > 0 and bfgs._n_cnot > 0
