@@ -30,12 +30,6 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# One hardware queue per serving lane (6) plus the library stream and the
-# caller's: with HIP's default of 4, lanes share queues and a short request
-# waits behind another lane's launch (half-load p50 13.6 ms against 2.0 ms).
-# Read when the HIP runtime starts, so it is set before anything touches the GPU.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per k_orbit_lanes<4, 2> launch at 30 qubits from rocprofv3 --pmc

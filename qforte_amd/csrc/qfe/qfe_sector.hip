@@ -284,45 +284,74 @@ __device__ __forceinline__ void load_words(uint2 (&w)[K], const uint2* __restric
     }
 }
 
-// The entries in order from pair words, K pairs per thread per chunk, with
-// the words of the next chunk (of this entry or the next) loaded while this
-// one is rotated: a step's dependent chain is the amplitude loads and stores.
+// Position of a chunk of K * THREADS pairs: entry e, first pair base (e ==
+// ne: past the last entry).
+struct WordCursor {
+    uint32_t e, base;
+};
+
 template <int K, int THREADS>
+__device__ __forceinline__ WordCursor next_chunk(const SecEntry* __restrict__ ent, uint32_t ne, WordCursor c) {
+    if (c.e >= ne) return c;
+    c.base += K * THREADS;
+    if (c.base >= ent[c.e].la_len * ent[c.e].lb_len) {
+        ++c.e;
+        c.base = 0;
+    }
+    return c;
+}
+
+template <int K, int THREADS>
+__device__ __forceinline__ void load_chunk(uint2 (&w)[K], const SecEntry* __restrict__ ent, const uint2* __restrict__ words,
+                                           uint32_t ne, WordCursor c, uint32_t tid) {
+    if (c.e < ne) load_words<K, THREADS>(w, words + ent[c.e].w_off, ent[c.e].la_len * ent[c.e].lb_len, c.base, tid);
+}
+
+// The entries in order from pair words, K pairs per thread per chunk.  The
+// words of the chunk DEPTH ahead are loaded while the current one rotates, so
+// a step's dependent chain is the amplitude loads and stores (DEPTH = 2 when
+// the state is in LDS and a step is shorter than a load from memory).
+template <int K, int THREADS, int DEPTH>
 __device__ __forceinline__ void ansatz_words(const SlotDesc& sd, const SecTables* __restrict__ T, double2* __restrict__ psi,
                                              uint32_t tid) {
-    constexpr uint32_t CH = K * THREADS;
+    static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth 1 or 2");
     const SecEntry* __restrict__ ent = T->entries;
     const uint2* __restrict__ words = T->words;
     const uint32_t ne = T->nentries;
-    uint2 w[K];
-    if (ne) load_words<K, THREADS>(w, words + ent[0].w_off, ent[0].la_len * ent[0].lb_len, 0, tid);
+    WordCursor cur{0u, 0u};
+    WordCursor c1 = next_chunk<K, THREADS>(ent, ne, cur);
+    uint2 w[K], w1[K], w2[K];
+    load_chunk<K, THREADS>(w, ent, words, ne, cur, tid);
+    if constexpr (DEPTH == 2) load_chunk<K, THREADS>(w1, ent, words, ne, c1, tid);
+    WordCursor c2 = next_chunk<K, THREADS>(ent, ne, c1);
     __syncthreads();
-    for (uint32_t e = 0; e < ne; ++e) {
-        const uint32_t P = ent[e].la_len * ent[e].lb_len, woff = ent[e].w_off;
+    while (cur.e < ne) {
+        const uint32_t e = cur.e, P = ent[e].la_len * ent[e].lb_len;
         const double2 cs = sd.cs[e], ulo = ent[e].ulo, uhi = ent[e].uhi;
-        for (uint32_t base = 0; base < P; base += CH) {
-            double2 a[K], b[K];
+        double2 a[K], b[K];
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (base + k * THREADS + tid < P) {
-                    a[k] = psi[w[k].x & kWordRank];
-                    b[k] = psi[w[k].y & kWordRank];
-                }
-            }
-            uint2 nw[K];
-            if (base + CH < P)
-                load_words<K, THREADS>(nw, words + woff, P, base + CH, tid);
-            else if (e + 1 < ne)
-                load_words<K, THREADS>(nw, words + ent[e + 1].w_off, ent[e + 1].la_len * ent[e + 1].lb_len, 0, tid);
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (base + k * THREADS + tid < P)
-                    pair_rotate_signed(psi, w[k].x & kWordRank, w[k].y & kWordRank, a[k], b[k], cs.x,
-                                       word_sign(w[k].y, cs.y), word_sign(w[k].x, cs.y), ulo, uhi);
-                w[k] = nw[k];
+        for (int k = 0; k < K; ++k) {
+            if (cur.base + k * THREADS + tid < P) {
+                a[k] = psi[w[k].x & kWordRank];
+                b[k] = psi[w[k].y & kWordRank];
             }
         }
-        if (e + 1 == ne || ent[e + 1].step != ent[e].step) __syncthreads();
+        if constexpr (DEPTH == 2)
+            load_chunk<K, THREADS>(w2, ent, words, ne, c2, tid);
+        else
+            load_chunk<K, THREADS>(w1, ent, words, ne, c1, tid);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (cur.base + k * THREADS + tid < P)
+                pair_rotate_signed(psi, w[k].x & kWordRank, w[k].y & kWordRank, a[k], b[k], cs.x,
+                                   word_sign(w[k].y, cs.y), word_sign(w[k].x, cs.y), ulo, uhi);
+            w[k] = w1[k];
+            if constexpr (DEPTH == 2) w1[k] = w2[k];
+        }
+        if (cur.base + K * THREADS >= P && (e + 1 == ne || ent[e + 1].step != ent[e].step)) __syncthreads();
+        cur = c1;
+        c1 = c2;
+        c2 = next_chunk<K, THREADS>(ent, ne, c2);
     }
 }
 
@@ -336,7 +365,7 @@ __global__ __launch_bounds__(THREADS) void k_sec_ansatz(const SlotDesc* __restri
     double2* __restrict__ psi = PSI_LDS ? sh : sd.psi;
     for (uint64_t r = tid; r < D; r += THREADS) psi[r] = make_double2(r == T->ref_rank ? 1.0 : 0.0, 0.0);
     if (T->words) {
-        ansatz_words<K, THREADS>(sd, T, psi, tid);
+        ansatz_words<K, THREADS, PSI_LDS ? 2 : 1>(sd, T, psi, tid);
     } else {
         const uint64_t *af, *bf;
         stage_tables(T, (uint64_t*)(sh + (PSI_LDS ? D : 0)), af, bf, tid, THREADS);
