@@ -200,12 +200,17 @@ __device__ __forceinline__ void pair_rotate(double2* __restrict__ psi, uint64_t 
 constexpr uint32_t kWordRank = 0x7fffffffu;
 __device__ __forceinline__ double word_sign(uint32_t v, double s) { return (v >> 31) ? -s : s; }
 
-// <sigma|K|psi> contribution of one pair (the entry_overlap formula).
-__device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, double2 pa, double2 pb, double2 sa,
-                                               double2 sb) {
-    const double s1 = en.w * parity_sign(j1 & en.z0), s2 = en.w * parity_sign((j1 ^ en.x) & en.z0);
+// <sigma|K|psi> contribution of one pair (the entry_overlap formula), with
+// s1 = w (-1)^{|j1 & z0|} and s2 = w (-1)^{|(j1 ^ x) & z0|}.
+__device__ __forceinline__ double pair_overlap_signed(const SecEntry& en, double s1, double s2, double2 pa, double2 pb,
+                                                      double2 sa, double2 sb) {
     const double2 k1 = cmul(en.uhi, pb), k2 = cmul(en.ulo, pa);
     return s2 * (sa.x * k1.x + sa.y * k1.y) + s1 * (sb.x * k2.x + sb.y * k2.y);
+}
+__device__ __forceinline__ double pair_overlap(const SecEntry& en, uint64_t j1, double2 pa, double2 pb, double2 sa,
+                                               double2 sb) {
+    return pair_overlap_signed(en, en.w * parity_sign(j1 & en.z0), en.w * parity_sign((j1 ^ en.x) & en.z0), pa, pb, sa,
+                               sb);
 }
 
 // v with its sign bit flipped when neg = 1u << 31: adding flip_sign(c, neg) is
@@ -602,6 +607,28 @@ __global__ __launch_bounds__(THREADS) void k_sec_reverse(const SlotDesc* __restr
     for (uint32_t k = T->ngen; k-- > 0;) {
         const uint32_t s0 = T->gen_step[k], s1 = T->gen_step[k + 1];
         double acc = 0.0;
+        if (s1 == s0 + 1 && T->words) {
+            // one step from pair words: each pair's four old values give its
+            // overlap term and its un-rotated psi and sigma pair in one pass
+            for (uint32_t en = T->steps[s0].e_first; en < T->steps[s0].e_first + T->steps[s0].e_count; ++en) {
+                const SecEntry& E = T->entries[en];
+                const double2 cs = sd.cs[en];
+                const uint2* __restrict__ wp = T->words + E.w_off;
+                const uint32_t total = E.la_len * E.lb_len;
+                for (uint32_t i = tid; i < total; i += THREADS) {
+                    const uint2 w = wp[i];
+                    const uint64_t r1 = w.x & kWordRank, r2 = w.y & kWordRank;
+                    const double2 p1 = psi[r1], p2 = psi[r2], g1 = sig[r1], g2 = sig[r2];
+                    acc += pair_overlap_signed(E, word_sign(w.y, E.w), word_sign(w.x, E.w), p1, p2, g1, g2);
+                    const double n1 = word_sign(w.y, -cs.y), n2 = word_sign(w.x, -cs.y);
+                    pair_rotate_signed(psi, r1, r2, p1, p2, cs.x, n1, n2, E.ulo, E.uhi);
+                    pair_rotate_signed(sig, r1, r2, g1, g2, cs.x, n1, n2, E.ulo, E.uhi);
+                }
+            }
+            const double2 tot = block_sum(make_double2(acc, 0.0), red);  // its barriers order the writes above
+            if (tid == 0) sd.gacc[k] = tot.x;
+            continue;
+        }
         if (s1 == s0 + 1) {
             // one step: each pair's four old values give its overlap term and its
             // un-rotated psi and sigma pair in one pass (pairs of a step are disjoint)
