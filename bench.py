@@ -86,6 +86,23 @@ def time_single_sweep(qf, W, qc, n, reps=40):
     return ms / reps
 
 
+def time_sampling(qf, qc, S, draws=1 << 20):
+    """Computational-basis draws from the resident state (measure_circuit's
+    sampling step, computer.cc:166-188): one probability scan of the state
+    (S bytes read) plus one wave per draw; wall clock per call, indices
+    included in host memory."""
+    qc.sample(draws, 1)
+    qf.synchronize()
+    t0 = time.perf_counter()
+    for k in range(3):
+        idx = qc.sample(draws, 2 + k)
+    dt = (time.perf_counter() - t0) / 3
+    assert int(idx.max()) < (1 << qc.get_nqubit())
+    return {"kernels": "k_sample_scan + k_sample_chunks + k_sample_draw", "draws": draws,
+            "ms_per_call": round(dt * 1e3, 3), "scan_bytes": S,
+            "effective_GBps": round(S / dt / 1e9, 1), "draws_per_s": round(draws / dt, 1)}
+
+
 def time_single_gate_sweep(qf, qc, n, reps=40, seed=105):
     """Average duration of a dense one-qubit gate sweep (Ry then Rz merged
     into one 2x2 on a random qubit, one gate per pass), HIP events."""
@@ -584,6 +601,7 @@ def main():
             "attainable_in_place_sweep": {"kernel": "k_scale (reads and writes every amplitude once)",
                                           "avg_launch_ms": round(copy_ms, 4),
                                           "achieved": round(2 * S / (copy_ms * 1e-3) / 1e9, 1)},
+            "sampling": time_sampling(qf, qc, S),
         }
 
     cpu = None

@@ -26,6 +26,16 @@ bool in(const std::string& s, const char* (&list)[N]) {
     return std::any_of(std::begin(list), std::end(list), [&](const char* t) { return s == t; });
 }
 
+// Basis indices drawn from |psi|^2 on the device (qfe_sample), seeded.
+py::array_t<uint64_t> sample_indices(const Computer& c, size_t n, uint64_t seed) {
+    py::array_t<uint64_t> out(n);
+    {
+        py::gil_scoped_release rel;
+        check(qfe_sample(c.handle(), n, seed, out.mutable_data()), "Computer.sample");
+    }
+    return out;
+}
+
 py::array_t<std::complex<double>> coeff_array(const Computer& c) {
     py::array_t<std::complex<double>> out(c.get_nbasis());
     {
@@ -482,6 +492,7 @@ PYBIND11_MODULE(_core, m) {
         .def("direct_pauli_circ_exp_val", &Computer::direct_pauli_circ_exp_val)
         .def("coeff", &Computer::coeff)
         .def("get_coeff_vec", &coeff_array)
+        .def("sample", &sample_indices, "n_measurements"_a, "seed"_a)
         .def("get_nqubit", &Computer::get_nqubit)
         .def("set_coeff_vec", &set_coeffs_from)
         .def("set_coeff_vec_from_numpy", &set_coeffs_from)

@@ -159,3 +159,16 @@ def test_zero_norm_state_is_an_error(lib):
     out = np.zeros(4, dtype=np.uint64)
     assert lib.qfe_sample(s.h, ctypes.c_uint64(4), ctypes.c_uint64(1), out.ctypes.data_as(ctypes.c_void_p)) != 0
     assert b"zero norm" in lib.qfe_last_error()
+
+
+def test_computer_sample_binding_is_seeded_and_in_range(qf):
+    comp = qf.Computer(12)
+    for q in range(12):
+        comp.apply_gate(qf.gate("H", q, q))
+    a = comp.sample(4096, 99)
+    b = comp.sample(4096, 99)
+    assert a.dtype == np.uint64 and a.shape == (4096,)
+    assert (a == b).all() and int(a.max()) < (1 << 12)
+    # uniform superposition: every outcome equally likely
+    counts = np.bincount(a.astype(np.int64), minlength=1 << 12)
+    assert counts.max() < 12
