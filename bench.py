@@ -98,7 +98,7 @@ def time_sampling(qf, qc, S, draws=1 << 20):
         idx = qc.sample(draws, 2 + k)
     dt = (time.perf_counter() - t0) / 3
     assert int(idx.max()) < (1 << qc.get_nqubit())
-    return {"kernels": "k_sample_scan + k_sample_chunks + k_sample_draw", "draws": draws,
+    return {"kernels": "k_sample_scan + k_sample_chunks + k_sample_guide + k_sample_draw", "draws": draws,
             "ms_per_call": round(dt * 1e3, 3), "scan_bytes": S,
             "effective_GBps": round(S / dt / 1e9, 1), "draws_per_s": round(draws / dt, 1)}
 

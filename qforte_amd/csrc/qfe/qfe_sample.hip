@@ -11,11 +11,12 @@
 //     state (16 bytes per amplitude, HBM-bound).
 //  2. k_sample_chunks: one workgroup turns the chunk masses into an inclusive
 //     prefix (2^(n-14) values).
-//  3. k_sample_draw: one wave per sample.  u = U[0,1) * total from a
+//  3. k_sample_guide: a guide table over the chunk prefix (one entry per chunk).
+//  4. k_sample_draw: one wave per sample.  u = U[0,1) * total from a
 //     counter-based generator (SplitMix64, Steele, Lea and Flood 2014, applied
-//     to seed + (i + 1) * golden ratio), then a binary search over the chunk
-//     prefix, one over the chunk's sub-block prefix and a wave scan over the
-//     sub-block's 64 probabilities.
+//     to seed + (i + 1) * golden ratio), then a binary search over the few
+//     chunks the guide table leaves, one over the chunk's sub-block prefix and
+//     a wave scan over the sub-block's 64 probabilities.
 // The first position whose prefix exceeds u always has positive mass; if
 // rounding leaves u at or above a level's total, the last position with
 // positive mass at that level is taken.
@@ -134,13 +135,13 @@ __device__ __forceinline__ double uniform01(uint64_t seed, uint64_t i) {
     return (double)(z >> 11) * 0x1.0p-53;
 }
 
-// First index in [0, n) of a non-decreasing prefix array whose value exceeds
-// u; if none does, the first index that reaches the last value (the last
-// position with positive mass).
-__device__ __forceinline__ uint64_t prefix_find(const double* __restrict__ v, uint64_t n, double u) {
+// First index in [lo, hi] of a non-decreasing prefix array whose value
+// exceeds u; if none does, the first index that reaches the last value (the
+// last position with positive mass).  The answer must lie in [lo, hi].
+__device__ __forceinline__ uint64_t prefix_find_in(const double* __restrict__ v, uint64_t n, double u, uint64_t lo,
+                                                   uint64_t hi) {
     const double top = v[n - 1];
     const bool over = !(u < top);
-    uint64_t lo = 0, hi = n - 1;
     while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         if (over ? !(v[mid] < top) : v[mid] > u)
@@ -150,17 +151,45 @@ __device__ __forceinline__ uint64_t prefix_find(const double* __restrict__ v, ui
     }
     return lo;
 }
+__device__ __forceinline__ uint64_t prefix_find(const double* __restrict__ v, uint64_t n, double u) {
+    return prefix_find_in(v, n, u, 0, n - 1);
+}
+
+// Guide table of the chunk prefix (indexed search, Chen and Asau 1974):
+// guide[b] = first chunk whose prefix exceeds b * total / nguide.  A draw u
+// with b = floor(u * nguide / total) has its chunk in [guide[b - 1],
+// guide[b + 2]] even when that floor is off by one in rounding.
+__global__ __launch_bounds__(256) void k_sample_guide(const double* __restrict__ chunk_prefix, uint64_t nchunk,
+                                                      uint32_t* __restrict__ guide, uint32_t nguide) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nguide) return;
+    const double target = chunk_prefix[nchunk - 1] * ((double)b / (double)nguide);
+    uint64_t lo = 0, hi = nchunk - 1;
+    while (lo < hi) {  // first c with prefix[c] > target (nchunk - 1 if none)
+        const uint64_t mid = (lo + hi) >> 1;
+        if (chunk_prefix[mid] > target)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    guide[b] = (uint32_t)lo;
+}
 
 __global__ __launch_bounds__(kDrawWaves * kWave) void k_sample_draw(const double2* __restrict__ psi, SampleShape sh,
                                                                      const double* __restrict__ sub_prefix,
                                                                      const double* __restrict__ chunk_prefix,
+                                                                     const uint32_t* __restrict__ guide, uint32_t nguide,
                                                                      uint64_t nsamples, uint64_t seed,
                                                                      uint64_t* __restrict__ out) {
     const uint32_t lane = threadIdx.x % kWave;
     const uint64_t i = (uint64_t)blockIdx.x * kDrawWaves + threadIdx.x / kWave;
     if (i >= nsamples) return;  // whole waves leave together
-    double u = uniform01(seed, i) * chunk_prefix[sh.nchunk - 1];
-    const uint64_t c = prefix_find(chunk_prefix, sh.nchunk, u);
+    const double r = uniform01(seed, i), total = chunk_prefix[sh.nchunk - 1];
+    double u = r * total;
+    const uint32_t b = min((uint32_t)(r * (double)nguide), nguide - 1);
+    const uint64_t glo = guide[b > 0 ? b - 1 : 0];
+    const uint64_t ghi = b + 2 < nguide ? guide[b + 2] : sh.nchunk - 1;
+    const uint64_t c = u < total ? prefix_find_in(chunk_prefix, sh.nchunk, u, glo, ghi) : prefix_find(chunk_prefix, sh.nchunk, u);
     if (c) u -= chunk_prefix[c - 1];
     const double* sp = sub_prefix + c * sh.chunk_subs;
     const uint64_t s = prefix_find(sp, sh.chunk_subs, u);
@@ -195,13 +224,16 @@ int qfe_sample(qfe_state s, uint64_t nsamples, uint64_t seed, uint64_t* out) {
     int rc = ensure_init();
     if (rc) return rc;
     const SampleShape sh = sample_shape(s->N);
+    const uint32_t nguide = (uint32_t)sh.nchunk;
     const size_t o_chunk = (size_t)sh.nsub * 8;
-    const size_t o_out = o_chunk + ((size_t)sh.nchunk * 8 + 255) / 256 * 256;
+    const size_t o_guide = o_chunk + ((size_t)sh.nchunk * 8 + 255) / 256 * 256;
+    const size_t o_out = o_guide + ((size_t)nguide * 4 + 255) / 256 * 256;
     void* scratch = nullptr;
     rc = scratch_reserve(o_out + (size_t)nsamples * 8, &scratch);
     if (rc) return rc;
     double* sub_prefix = (double*)scratch;
     double* chunk_prefix = (double*)((char*)scratch + o_chunk);
+    uint32_t* guide = (uint32_t*)((char*)scratch + o_guide);
     uint64_t* dout = (uint64_t*)((char*)scratch + o_out);
     hipStream_t st = ctx().stream;
     hipLaunchKernelGGL(k_sample_scan, dim3((unsigned)sh.nchunk), dim3(kScanThreads), 0, st, s->d, sh, sub_prefix,
@@ -213,9 +245,11 @@ int qfe_sample(qfe_state s, uint64_t nsamples, uint64_t seed, uint64_t* out) {
     QFE_HIP(hipMemcpyAsync(&total, chunk_prefix + sh.nchunk - 1, 8, hipMemcpyDeviceToHost, st));
     QFE_HIP(hipStreamSynchronize(st));
     if (!(total > 0.0)) return fail(QFE_ERR_ARG, "qfe_sample: the state has zero norm");
+    hipLaunchKernelGGL(k_sample_guide, dim3((nguide + 255) / 256), dim3(256), 0, st, chunk_prefix, sh.nchunk, guide, nguide);
+    QFE_HIP(hipGetLastError());
     const uint64_t blocks = (nsamples + kDrawWaves - 1) / kDrawWaves;
     hipLaunchKernelGGL(k_sample_draw, dim3((unsigned)blocks), dim3(kDrawWaves * kWave), 0, st, s->d, sh, sub_prefix,
-                       chunk_prefix, nsamples, seed, dout);
+                       chunk_prefix, guide, nguide, nsamples, seed, dout);
     QFE_HIP(hipGetLastError());
     QFE_HIP(hipMemcpyAsync(out, dout, nsamples * 8, hipMemcpyDeviceToHost, st));
     QFE_HIP(hipStreamSynchronize(st));
