@@ -104,6 +104,11 @@ int qfe_event_destroy(void* event);
 /* Counters of launched passes since the last reset (fused passes, single-op
  * passes, expectation passes) and the algorithmic bytes they moved. */
 int qfe_get_counters(uint64_t* fused_passes, uint64_t* single_passes, uint64_t* reduce_passes, double* bytes);
+/* The same counters split by kind: out[0..count) of {fused passes, single-op
+ * passes, expectation passes, LDS-tile passes, register-orbit passes,
+ * lane-orbit passes, passes with non-temporal loads and stores}; no
+ * reference counterpart (tests and benchmarks read which kernels ran). */
+int qfe_get_pass_counts(uint64_t* out, uint64_t count);
 int qfe_reset_counters(void);
 
 /* ---- state (Computer storage, computer.cc:28-54, computer.h:28,112,132) -- */
@@ -246,6 +251,17 @@ int qfe_template_create(int nqubit, uint64_t ref_index, const qfe_pauli* gen_ter
 int qfe_template_destroy(qfe_template t);
 /* Sector dimension and device bytes one request of this template occupies. */
 int qfe_template_info(qfe_template t, uint64_t* sector_dim, uint64_t* request_bytes);
+/* Host-side audit of a captured single-request sequence (no reference
+ * counterpart; a diagnostic for callers and tests).  Templates whose state
+ * does not fit one workgroup's LDS replay their energy (gradient = 0) and
+ * gradient (gradient = 1) sequences as hipGraphs; every capture is checked
+ * before it is kept: one chain of nodes, each node the expected memcpy or
+ * kernel, grids non-zero, blocks of 1..1024 threads, arguments and copy
+ * extents equal to the addresses the template owns.  Builds the graph if
+ * needed and writes out4 = {nodes, kernel nodes, kernel nodes whose arguments
+ * were compared, memcpy nodes whose parameters were compared}; all zero for
+ * templates that launch without a graph. */
+int qfe_template_graph_audit(qfe_template t, int gradient, uint64_t* out4);
 /* One request in one launch (no batching): E = <ref|U^dag H U|ref>. */
 int qfe_template_energy(qfe_template t, const double* params, const double* inputs, double* energy);
 /* U|ref> scattered into a full 2^n state (for checks against the full-space path). */

@@ -16,10 +16,11 @@ the amplitude vector.  Conventions (reference ``src/qforte``):
 * ``direct_op_exp_val`` returns ``<psi| sum_k c_k circ_k |psi>`` with the bra
   conjugated (``computer.cc:753-771``, default non-OpenMP build).
 
-Every function names the reference lines it follows.  How each one is pinned
-(closed forms, brute-force Kronecker products, the reference's own known-answer
-vectors, the compiled reference core) is listed in ``oracle/README.md`` and
-exercised by ``tests/test_oracle.py``.
+Every function names the reference lines it follows.  Each one is pinned in
+``tests/test_oracle.py`` by closed forms, brute-force Kronecker products,
+``scipy.linalg.expm``, the reference's own known-answer vectors and the
+reference C++ core compiled from its sources (``oracle/Makefile`` ->
+``oracle/_ref``), with planted mistakes that those checks must catch.
 """
 
 from __future__ import annotations
@@ -382,25 +383,6 @@ def operator_matrix(n: int, op) -> np.ndarray:
 # Second quantisation: Jordan-Wigner (sq_operator.cc:106-169)
 # ---------------------------------------------------------------------------
 
-def _pauli_mul(a, b):
-    """Product of two Pauli dicts {qubit: letter} with phase (a applied after b
-    in operator order a*b)."""
-    table = {("X", "Y"): (1j, "Z"), ("Y", "X"): (-1j, "Z"), ("Y", "Z"): (1j, "X"),
-             ("Z", "Y"): (-1j, "X"), ("Z", "X"): (1j, "Y"), ("X", "Z"): (-1j, "Y")}
-    out = dict(a)
-    ph = 1 + 0j
-    for q, l in b.items():
-        if q not in out:
-            out[q] = l
-        elif out[q] == l:
-            del out[q]
-        else:
-            p, r = table[(out[q], l)]
-            ph *= p
-            out[q] = r
-    return ph, out
-
-
 def jw_dense(terms, n: int) -> np.ndarray:
     """Dense matrix of a fermionic operator via Jordan-Wigner, built from
     a_p = 1/2 (X_p + i Y_p) prod_{k<p} Z_k and a_p^dag = 1/2 (X_p - i Y_p)
@@ -441,8 +423,3 @@ def ucc_state(n: int, ref_occ: Sequence[int], exps) -> np.ndarray:
     for c, term in exps:
         psi = exp_pauli(psi, c, term)
     return psi
-
-
-def ucc_energy(n, ref_occ, exps, ham) -> float:
-    psi = ucc_state(n, ref_occ, exps)
-    return float(np.real(expval(psi, n, ham)))

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <unordered_map>
 
 #include "core.h"
 
@@ -241,6 +242,15 @@ class SectorTemplate {
         qfe_template_info(t_, nullptr, &b);
         return b;
     }
+    // {nodes, kernel nodes, kernel nodes with checked arguments, memcpy nodes with checked parameters}
+    std::vector<uint64_t> graph_audit(bool gradient) {
+        std::vector<uint64_t> out(4);
+        {
+            py::gil_scoped_release rel;
+            check(qfe_template_graph_audit(t_, gradient ? 1 : 0, out.data()), "SectorTemplate.graph_audit");
+        }
+        return out;
+    }
     int nqubit() const { return n_; }
     size_t nparam() const { return nparam_; }
     size_t ninputs() const { return ninputs_; }
@@ -287,7 +297,7 @@ class BatchEngine {
             check(qfe_batch_submit_gradient(b_, t->handle(), amps.data(), inputs.data(), &id), "BatchEngine.submit");
         else
             check(qfe_batch_submit(b_, t->handle(), amps.data(), inputs.data(), &id), "BatchEngine.submit");
-        keep_.push_back(t);
+        keep_.emplace(id, std::move(t));
         return id;
     }
     std::vector<double> take_gradient(uint64_t id, size_t nparam) {
@@ -316,14 +326,15 @@ class BatchEngine {
                 check(qfe_batch_poll(b_, ids.data(), es.data(), cap, &nd), "BatchEngine.poll");
         }
         std::vector<std::pair<uint64_t, double>> out;
-        for (uint64_t i = 0; i < nd; ++i) out.emplace_back(ids[i], es[i]);
-        uint64_t q = 0, a = 0;
-        qfe_batch_pending(b_, &q, &a);
-        if (q == 0 && a == 0) keep_.clear();
+        for (uint64_t i = 0; i < nd; ++i) {
+            out.emplace_back(ids[i], es[i]);
+            keep_.erase(ids[i]);  // the engine no longer reads the request's template
+        }
         return out;
     }
     qfe_batch b_ = nullptr;
-    std::vector<std::shared_ptr<SectorTemplate>> keep_;
+    // the template of every queued or running request, released when its result is returned
+    std::unordered_map<uint64_t, std::shared_ptr<SectorTemplate>> keep_;
 };
 
 // HIP-event timer on the engine stream.
@@ -664,6 +675,7 @@ PYBIND11_MODULE(_core, m) {
              "inputs"_a = std::vector<double>())
         .def_property_readonly("dim", &SectorTemplate::dim)
         .def_property_readonly("request_bytes", &SectorTemplate::request_bytes)
+        .def("graph_audit", &SectorTemplate::graph_audit, "gradient"_a = false)
         .def_property_readonly("nqubit", &SectorTemplate::nqubit)
         .def_property_readonly("nparam", &SectorTemplate::nparam)
         .def_property_readonly("ninputs", &SectorTemplate::ninputs);
@@ -699,10 +711,13 @@ PYBIND11_MODULE(_core, m) {
         return py::make_tuple(k, l, o);
     });
     m.def("get_counters", []() {
-        uint64_t f, s, r;
+        uint64_t f, s, r, v[7];
         double b;
         qfe_get_counters(&f, &s, &r, &b);
-        return py::dict("fused_passes"_a = f, "single_passes"_a = s, "reduce_passes"_a = r, "bytes"_a = b);
+        qfe_get_pass_counts(v, 7);
+        return py::dict("fused_passes"_a = f, "single_passes"_a = s, "reduce_passes"_a = r, "bytes"_a = b,
+                        "tile_passes"_a = v[3], "orbit_passes"_a = v[4], "lane_orbit_passes"_a = v[5],
+                        "nt_passes"_a = v[6]);
     });
     m.def("reset_counters", []() { qfe_reset_counters(); });
     m.def("synchronize", []() { check(qfe_synchronize(), "synchronize"); });

@@ -292,7 +292,34 @@ std::vector<cplx> Computer::direct_idxd_oppl_exp_val(const QubitOpPool& pool, co
 }
 
 std::vector<cplx> Computer::direct_oppl_exp_val_w_mults(const QubitOpPool& pool, const std::vector<cplx>& mults) {
-    // computer.cc:808-835 (default build): sum_l mults[l] c_pool c_l <circ_l>
+    // computer.cc:808-835 (default build): sum_l mults[l] c_pool c_l <circ_l>.
+    // Pools of Pauli sums go to the device in one call: term l of operator k
+    // carries the coefficient mults[l] c_l, and c_pool scales the operator's sum.
+    std::vector<qfe_pauli> all, one;
+    std::vector<uint64_t> off{0};
+    bool pauli = true;
+    for (const auto& pt : pool.terms()) {
+        const auto& ts = pt.second.terms();
+        if (ts.size() > mults.size()) throw std::out_of_range("direct_oppl_exp_val_w_mults: too few multipliers");
+        if (!compile_pauli_sum(pt.second, one)) {
+            pauli = false;
+            break;
+        }
+        for (size_t l = 0; l < one.size(); ++l) {
+            const cplx c = mults[l] * cplx(one[l].coeff.re, one[l].coeff.im);
+            one[l].coeff = qfe_c64{c.real(), c.imag()};
+        }
+        all.insert(all.end(), one.begin(), one.end());
+        off.push_back(all.size());
+    }
+    if (pauli) {
+        const size_t nops = pool.terms().size();
+        std::vector<qfe_c64> r(nops);
+        if (nops) check(qfe_expval_many(s_, all.data(), off.data(), nops, r.data()), "direct_oppl_exp_val_w_mults");
+        std::vector<cplx> out;
+        for (size_t k = 0; k < nops; ++k) out.push_back(pool.terms()[k].first * cplx(r[k].re, r[k].im));
+        return out;
+    }
     std::vector<cplx> out;
     for (const auto& pt : pool.terms()) {
         cplx acc = 0.0;

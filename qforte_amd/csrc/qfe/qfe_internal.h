@@ -33,6 +33,9 @@ struct Tuning {
 
 struct Counters {
     uint64_t fused = 0, single = 0, reduce = 0;
+    // fused passes by kind, and read-write or expectation passes launched with
+    // non-temporal accesses
+    uint64_t tile = 0, orbit = 0, lane_orbit = 0, nt = 0;
     double bytes = 0.0;
 };
 

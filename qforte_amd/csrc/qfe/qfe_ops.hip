@@ -1043,6 +1043,8 @@ static int launch_orbit(qfe_state_s* st, int M, const OrbitDesc& d, const OrbitO
     }
     QFE_HIP(hipGetLastError());
     cx.counters.fused++;
+    cx.counters.orbit++;
+    cx.counters.nt += nt ? 1 : 0;
     cx.counters.bytes += 32.0 * (double)st->N;
     return QFE_OK;
 }
@@ -1153,6 +1155,8 @@ static int launch_orbit_lanes(qfe_state_s* st, int M, uint32_t low_pivot, const 
         return fail(QFE_ERR_STATE, "internal: lane orbit dimension out of range");
     QFE_HIP(hipGetLastError());
     cx.counters.fused++;
+    cx.counters.lane_orbit++;
+    cx.counters.nt += nt ? 1 : 0;
     cx.counters.bytes += 32.0 * (double)st->N;
     return QFE_OK;
 }
@@ -1193,6 +1197,8 @@ static int launch_fused(qfe_state_s* st, const PassDesc& pd, const DevOp* dops) 
     }
     QFE_HIP(hipGetLastError());
     cx.counters.fused++;
+    cx.counters.tile++;
+    cx.counters.nt += nt;
     cx.counters.bytes += 32.0 * (double)st->N;
     return QFE_OK;
 }

@@ -444,6 +444,7 @@ static int expval_groups_orbit(const double2* bra, const double2* ket, int n, co
         }
         QFE_HIP(hipGetLastError());
         cx.counters.reduce++;
+        cx.counters.nt += nt ? 1 : 0;
         cx.counters.bytes += (bra == ket ? 16.0 : 32.0) * (double)N;
     }
     hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)G), dim3(kEvTPB), 0, cx.stream, partial, nb, dout);

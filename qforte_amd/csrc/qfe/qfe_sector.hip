@@ -1078,6 +1078,9 @@ struct qfe_template_s {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     void* gpin = nullptr;
+    // host-side audit of each captured sequence: nodes, kernel nodes, kernel
+    // nodes whose arguments were compared, memcpy nodes whose parameters were compared
+    uint64_t audit[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 };
 
 namespace qfe {
@@ -1578,6 +1581,190 @@ int single_psi_sigma(qfe_template t, const double* params, const double* inputs,
 // generator overlaps) back into the same block.
 size_t single_head(qfe_template t) { return 768 + align256(t->io_count * 8 + 8); }
 
+// ---- host-side audit of a captured sequence ---------------------------------------------
+// After instantiation every node of a template graph is compared with the
+// sequence the template's structure fixes, computed here independently of the
+// wide_* launchers: the graph must be one chain from its root; each node must
+// be the expected memcpy (addresses and size inside the arena and the pinned
+// block) or the expected kernel with a non-zero grid and a block of at most
+// 1024 threads, every scalar argument equal to the expected one and every
+// pointer argument equal to the expected address (partial-sum pointers with
+// their whole grid inside the partial array); the uploaded descriptor's
+// pointers must lie inside the arena.  A mismatch fails graph_build with
+// QFE_ERR_STATE and the graph is not kept, so nothing malformed is launched.
+struct GraphAudit {
+    uint64_t nodes = 0, kernels = 0, kernels_checked = 0, copies_checked = 0;
+};
+
+struct NodeExpect {
+    bool copy = false;
+    const void* func = nullptr;
+    unsigned grid = 0, block = 0;
+    std::vector<std::vector<char>> args;  // expected bytes of each argument
+    const void* dst = nullptr;
+    const void* src = nullptr;
+    size_t bytes = 0;
+};
+
+template <typename... A>
+NodeExpect expect_kernel(const void* func, uint64_t grid, unsigned block, A... a) {
+    NodeExpect e;
+    e.func = func;
+    e.grid = grid > 0x7fffffffull ? 0u : (unsigned)grid;  // out of range: fails the non-zero grid check
+    e.block = block;
+    (e.args.push_back(std::vector<char>((const char*)&a, (const char*)&a + sizeof(A))), ...);
+    return e;
+}
+
+NodeExpect expect_copy(const void* dst, const void* src, size_t bytes) {
+    NodeExpect e;
+    e.copy = true;
+    e.dst = dst;
+    e.src = src;
+    e.bytes = bytes;
+    return e;
+}
+
+void expected_nodes(qfe_template t, int grad, std::vector<NodeExpect>& out) {
+    const size_t head = single_head(t);
+    char* dev = (char*)t->arena;
+    char* pin = (char*)t->gpin;
+    const SlotDesc* d = (const SlotDesc*)dev;
+    const auto KB = [](uint64_t items) { return (items + kWideTPB - 1) / kWideTPB; };
+    out.clear();
+    out.push_back(expect_copy(dev, pin, head));
+    out.push_back(expect_kernel((const void*)&k_sec_prologue, 1, 256, d, (int)1));
+    out.push_back(expect_kernel((const void*)&k_sec_ref, KB(t->D), kWideTPB, d));
+    for (uint32_t k = 0; k < (uint32_t)t->step_pairs.size(); ++k)
+        if (t->step_pairs[k])
+            out.push_back(expect_kernel((const void*)&k_sec_step, KB(t->step_pairs[k]), kWideTPB, d, k, (int)0, (int)0));
+    if (!grad) {
+        if (t->nitems)
+            out.push_back(expect_kernel((const void*)&k_sec_energy, t->nitems, kItemThreads, d,
+                                        (const uint32_t*)(dev + 512), (uint32_t)1));
+        out.push_back(expect_kernel((const void*)&k_sec_sum, 1, 256, d));
+        out.push_back(expect_copy(pin + head, dev + 256, 8));
+        return;
+    }
+    out.push_back(expect_kernel((const void*)&k_sec_hpsi<1>, (t->D + 255) / 256, 256, d));
+    double* part = (double*)((char*)t->wide + align256(t->wide_seg.size() * sizeof(uint2)));
+    out.push_back(expect_kernel((const void*)&k_sec_edot, t->wide_seg[0].y, kWideTPB, d, part + t->wide_seg[0].x));
+    for (uint32_t k = (uint32_t)t->ngen; k-- > 0;) {
+        const uint2 sg = t->wide_seg[k + 1];
+        if (!sg.y) continue;
+        const uint32_t s0 = t->h_gen_step[k], s1 = t->h_gen_step[k + 1];
+        if (s1 == s0 + 1) {
+            out.push_back(expect_kernel((const void*)&k_sec_genrev, sg.y, kWideTPB, d, s0, part + sg.x));
+            continue;
+        }
+        out.push_back(expect_kernel((const void*)&k_sec_genov, sg.y, kWideTPB, d, s0, s1, part + sg.x));
+        for (uint32_t sdx = s1; sdx-- > s0;)
+            if (t->step_pairs[sdx])
+                out.push_back(expect_kernel((const void*)&k_sec_step, KB(t->step_pairs[sdx]), kWideTPB, d, sdx, (int)1, (int)1));
+    }
+    const uint32_t nseg = (uint32_t)t->wide_seg.size();
+    out.push_back(expect_kernel((const void*)&k_sec_wsum, KB(nseg), kWideTPB, d, (const double*)part,
+                                (const uint2*)t->wide, nseg));
+    out.push_back(expect_copy(pin + head, dev + 256, 8));
+    if (t->ngen) out.push_back(expect_copy(pin + head + 8, dev + head + t->off_gacc, t->ngen * 8));
+}
+
+bool inside(const void* p, size_t bytes, const void* base, size_t cap) {
+    const uintptr_t a = (uintptr_t)p, b = (uintptr_t)base;
+    return a >= b && bytes <= cap && a - b <= cap - bytes;
+}
+
+int audit_fail(int grad, size_t node, const std::string& what) {
+    return fail(QFE_ERR_STATE, std::string("qfe_template: captured ") + (grad ? "gradient" : "energy") + " sequence, node " +
+                                   std::to_string(node) + ": " + what);
+}
+
+int audit_graph(qfe_template t, hipGraph_t g, int grad, GraphAudit& A) {
+    const size_t head = single_head(t);
+    const size_t arena_bytes = head + t->grad_bytes, pin_bytes = head + 8 * (t->ngen + 2);
+    const char* dev = (const char*)t->arena;
+    // descriptor as the first node uploads it
+    SlotDesc d;
+    std::memcpy(&d, t->gpin, sizeof(d));
+    const char* work = dev + head;
+    auto in_work = [&](const void* p, size_t bytes) { return p && inside(p, bytes, work, t->grad_bytes); };
+    if (d.t != t->dt || d.out != (double*)(dev + 256) || !in_work(d.psi, t->D * sizeof(double2)) ||
+        !in_work(d.sig, t->D * sizeof(double2)) || !in_work(d.cs, (size_t)t->nentries * sizeof(double2)) ||
+        !in_work(d.part, (size_t)t->nitems * 8) || !in_work(d.gacc, t->ngen * 8) ||
+        (t->has_map && !in_work(d.coef, (size_t)t->nterms * sizeof(double2))) ||
+        !inside(d.params, t->io_count * 8, dev + 768, head - 768))
+        return audit_fail(grad, 0, "descriptor pointer outside the arena");
+    std::vector<NodeExpect> want;
+    expected_nodes(t, grad, want);
+    size_t nroot = 0;
+    QFE_HIP(hipGraphGetRootNodes(g, nullptr, &nroot));
+    if (nroot != 1) return audit_fail(grad, 0, std::to_string(nroot) + " root nodes (expected one chain)");
+    hipGraphNode_t node = nullptr;
+    QFE_HIP(hipGraphGetRootNodes(g, &node, &nroot));
+    const double* part_lo = (const double*)((const char*)t->wide + align256(t->wide_seg.size() * sizeof(uint2)));
+    const double* part_hi = part_lo + t->wide_parts;
+    A = GraphAudit{};
+    for (size_t i = 0;; ++i) {
+        if (i >= want.size()) return audit_fail(grad, i, "more nodes than the sequence has launches");
+        const NodeExpect& w = want[i];
+        hipGraphNodeType type;
+        QFE_HIP(hipGraphNodeGetType(node, &type));
+        if (w.copy) {
+            if (type != hipGraphNodeTypeMemcpy) return audit_fail(grad, i, "expected a memcpy node");
+            // the expected copy itself must stay inside the arena and the pinned block
+            const bool up = w.dst == t->arena;
+            if (up ? !(inside(w.dst, w.bytes, t->arena, arena_bytes) && inside(w.src, w.bytes, t->gpin, pin_bytes))
+                   : !(inside(w.src, w.bytes, t->arena, arena_bytes) && inside(w.dst, w.bytes, t->gpin, pin_bytes)))
+                return audit_fail(grad, i, "memcpy outside the arena or the pinned block");
+            hipMemcpy3DParms p{};
+            // a runtime that does not read back captured 1D copies returns null pointers: shape unchecked
+            if (hipGraphMemcpyNodeGetParams(node, &p) == hipSuccess && (p.dstPtr.ptr || p.srcPtr.ptr)) {
+                // a captured 1D copy reads back as a width-only extent of `bytes`
+                const size_t bytes = p.extent.width * std::max<size_t>(1, p.extent.height) * std::max<size_t>(1, p.extent.depth);
+                if (p.dstPtr.ptr != w.dst || p.srcPtr.ptr != w.src || bytes != w.bytes)
+                    return audit_fail(grad, i, "memcpy addresses or size differ from the sequence");
+                ++A.copies_checked;
+            }
+        } else {
+            if (type != hipGraphNodeTypeKernel) return audit_fail(grad, i, "expected a kernel node");
+            hipKernelNodeParams p{};
+            QFE_HIP(hipGraphKernelNodeGetParams(node, &p));
+            if (p.func != w.func) return audit_fail(grad, i, "kernel differs from the sequence");
+            if (p.gridDim.x == 0 || p.gridDim.y != 1 || p.gridDim.z != 1 || w.grid == 0)
+                return audit_fail(grad, i, "zero or out-of-range grid");
+            if (p.blockDim.x == 0 || p.blockDim.x > 1024 || p.blockDim.y != 1 || p.blockDim.z != 1)
+                return audit_fail(grad, i, "block outside 1..1024 threads");
+            if (p.gridDim.x != w.grid || p.blockDim.x != w.block)
+                return audit_fail(grad, i, "launch shape differs from the sequence (grid " + std::to_string(p.gridDim.x) +
+                                               ", expected " + std::to_string(w.grid) + ")");
+            ++A.kernels;
+            // a runtime that keeps no argument copies leaves kernelParams null: shape checked only
+            for (size_t a = 0; p.kernelParams && a < w.args.size(); ++a) {
+                if (!p.kernelParams[a] || std::memcmp(p.kernelParams[a], w.args[a].data(), w.args[a].size()))
+                    return audit_fail(grad, i, "argument " + std::to_string(a) + " differs from the sequence");
+                if (w.args[a].size() == sizeof(void*) && w.func != (const void*)&k_sec_wsum) {
+                    const double* q;
+                    std::memcpy(&q, w.args[a].data(), sizeof(q));
+                    // a partial-sum pointer covers one double per workgroup of its launch
+                    if (q >= part_lo && q <= part_hi && q + p.gridDim.x > part_hi)
+                        return audit_fail(grad, i, "partial sums past the end of the partial array");
+                }
+            }
+            if (p.kernelParams) ++A.kernels_checked;
+        }
+        size_t ndep = 0;
+        QFE_HIP(hipGraphNodeGetDependentNodes(node, nullptr, &ndep));
+        if (ndep == 0) {
+            if (i + 1 != want.size()) return audit_fail(grad, i, "chain ends before the sequence does");
+            A.nodes = i + 1;
+            break;
+        }
+        if (ndep != 1) return audit_fail(grad, i, "node with " + std::to_string(ndep) + " dependents (expected one chain)");
+        QFE_HIP(hipGraphNodeGetDependentNodes(node, &node, &ndep));
+    }
+    return QFE_OK;
+}
+
 int graph_build(qfe_template t, int grad) {
     const size_t head = single_head(t);
     if (!t->arena) QFE_HIP(hipMalloc(&t->arena, head + t->grad_bytes));
@@ -1611,7 +1798,18 @@ int graph_build(qfe_template t, int grad) {
         const hipError_t ie = hipGraphInstantiate(&t->gexec[grad], g, nullptr, nullptr, 0);
         if (ie != hipSuccess) rc = hip_fail(ie, "qfe_template: instantiate");
     }
+    if (!rc) {
+        GraphAudit A;
+        rc = audit_graph(t, g, grad, A);
+        if (!rc) {
+            t->audit[grad][0] = A.nodes;
+            t->audit[grad][1] = A.kernels;
+            t->audit[grad][2] = A.kernels_checked;
+            t->audit[grad][3] = A.copies_checked;
+        }
+    }
     if (rc) {
+        if (t->gexec[grad]) (void)hipGraphExecDestroy(t->gexec[grad]);
         if (g) (void)hipGraphDestroy(g);
         t->gexec[grad] = nullptr;
         return rc;
@@ -1932,6 +2130,18 @@ int qfe_template_info(qfe_template t, uint64_t* sector_dim, uint64_t* request_by
     if (!t) return fail(QFE_ERR_ARG, "qfe_template_info: invalid handle");
     if (sector_dim) *sector_dim = t->D;
     if (request_bytes) *request_bytes = t->request_bytes;
+    return QFE_OK;
+}
+
+int qfe_template_graph_audit(qfe_template t, int gradient, uint64_t* out4) {
+    if (!t || !out4 || (gradient != 0 && gradient != 1)) return fail(QFE_ERR_ARG, "qfe_template_graph_audit: invalid argument");
+    for (int k = 0; k < 4; ++k) out4[k] = 0;
+    if (t->tier != 2) return QFE_OK;  // one-workgroup sequences are launched directly, never captured
+    if (!t->gexec[gradient]) {
+        int rc = graph_build(t, gradient);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < 4; ++k) out4[k] = t->audit[gradient][k];
     return QFE_OK;
 }
 

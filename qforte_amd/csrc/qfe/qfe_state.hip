@@ -433,6 +433,14 @@ int qfe_get_counters(uint64_t* fused_passes, uint64_t* single_passes, uint64_t* 
     return QFE_OK;
 }
 
+int qfe_get_pass_counts(uint64_t* out, uint64_t count) {
+    if (count && !out) return fail(QFE_ERR_ARG, "qfe_get_pass_counts: null output");
+    const Counters& c = g_ctx.counters;
+    const uint64_t v[7] = {c.fused, c.single, c.reduce, c.tile, c.orbit, c.lane_orbit, c.nt};
+    for (uint64_t i = 0; i < count && i < 7; ++i) out[i] = v[i];
+    return QFE_OK;
+}
+
 int qfe_reset_counters(void) {
     g_ctx.counters = Counters{};
     return QFE_OK;

@@ -235,6 +235,117 @@ def test_trotter_known_answer(qf):
     assert c[12].imag == pytest.approx(0.25349397560041553, abs=1e-15)
 
 
+@pytest.mark.parametrize("n", [5, 12])
+def test_trotter_number_two(qf, oracle, n):
+    """trotterize with trotter_number N = 2 (trotterization.py:37-56): every
+    term repeats N times with coefficient / N, the whole term list once per
+    repetition.  The terms do not commute, so a wrong order or a missing
+    division by N shows in the amplitudes."""
+    strings = W.random_pauli_strings(n, 12, seed=61, wmin=1, wmax=min(5, n))
+    op = W.exp_operator(qf, strings)
+    U1, _ = qf.trotterize(op)
+    U2, phase = qf.trotterize(op, trotter_number=2)
+    assert phase == 1.0
+    assert U2.size() == 2 * U1.size()
+    assert U2.num_exponential_segments() == 2 * len(strings)
+    psi = rand_state(n, 13)
+    qc = qf.Computer(n)
+    qc.set_coeff_vec(psi)
+    qc.apply_circuit(U2)
+    expect = psi.copy()
+    for _ in range(2):
+        for theta, term in strings:
+            expect = oracle.exp_pauli(expect, -0.5j * theta, term)
+    assert np.max(np.abs(qc.get_coeff_vec() - expect)) < AMP_TOL
+    once = psi.copy()
+    for theta, term in strings:
+        once = oracle.exp_pauli(once, -1j * theta, term)
+    assert np.max(np.abs(expect - once)) > 1e-3  # the N = 2 product is a different operator
+    with pytest.raises(NotImplementedError):
+        qf.trotterize(op, trotter_order=2)
+
+
+def test_controlled_trotterization_known_answer(qf):
+    """trotterize_w_cRz (trotterization.py:60-128) with the reference's own
+    known answers (tests/test_trotter.py:48-100): the ancilla in |1> applies
+    the exponential, in |0> leaves the register alone."""
+    op = qf.QubitOperator()
+    op.add(-1.0719145972781818j, qf.build_circuit("Y_0 X_1"))
+    op.add(1.0719145972781818j, qf.build_circuit("X_0 Y_1"))
+    U, phase = qf.trotterize_w_cRz(op, 2)
+    assert phase == 1.0
+    qc = qf.Computer(3)
+    qc.apply_gate(qf.gate("X", 0, 0))
+    qc.apply_gate(qf.gate("X", 2, 2))
+    qc.apply_circuit(U)
+    c = qc.get_coeff_vec()
+    assert c[5] == pytest.approx(-0.5421829373021542, abs=1e-15)
+    assert c[6] == pytest.approx(-0.8402604730072732, abs=1e-15)
+    qc = qf.Computer(3)
+    qc.apply_gate(qf.gate("X", 0, 0))
+    qc.apply_circuit(U)
+    assert qc.get_coeff_vec()[1] == pytest.approx(1, abs=1e-15)
+
+
+@pytest.mark.parametrize("open_control", [False, True])
+@pytest.mark.parametrize("trotter_number", [1, 2])
+def test_controlled_trotterization_against_oracle(qf, oracle, open_control, trotter_number):
+    """The controlled product on a random state: on the ancilla's active half
+    (|1>, or |0> with the open control of exponentiate.py:71-77) the register
+    gets the Trotter product, the other half is unchanged."""
+    n, anc = 7, 3
+    strings = W.random_pauli_strings(n - 1, 10, seed=67, wmin=1, wmax=4)
+    strings = [(th, [(l, q + (q >= anc)) for l, q in term]) for th, term in strings]  # skip the ancilla qubit
+    op = W.exp_operator(qf, strings)
+    U, _ = qf.trotterize_w_cRz(op, anc, Use_open_cRz=open_control, trotter_number=trotter_number)
+    psi = rand_state(n, 71)
+    qc = qf.Computer(n)
+    qc.set_coeff_vec(psi)
+    qc.apply_circuit(U)
+    full = psi.copy()
+    for _ in range(trotter_number):
+        for theta, term in strings:
+            full = oracle.exp_pauli(full, -1j * theta / trotter_number, term)
+    active = ((np.arange(1 << n) >> anc) & 1) == (0 if open_control else 1)
+    expect = np.where(active, full, psi)
+    assert np.max(np.abs(qc.get_coeff_vec() - expect)) < AMP_TOL
+
+
+def test_pool_expectations_with_multipliers(qf, oracle):
+    """direct_oppl_exp_val_w_mults (computer.cc:808-835, default build):
+    entry k = sum_l mults[l] c_k c_kl <psi|P_kl|psi>, one device call for the
+    whole pool; a non-Pauli term takes the gate path for that pool."""
+    n = 9
+    psi = rand_state(n, 17)
+    qc = qf.Computer(n)
+    qc.set_coeff_vec(psi)
+    pool = qf.QubitOpPool()
+    subs = []
+    for k in range(6):
+        sub = W.random_pauli_sum(n, 3 + k, seed=200 + k, wmax=5, real_coeffs=False)
+        subs.append(sub)
+        pool.add(0.3 - 0.2j * k, W.to_operator(qf, sub))
+    mults = list(np.random.default_rng(5).normal(size=12) + 1j * np.random.default_rng(6).normal(size=12))
+    vals = qc.direct_oppl_exp_val_w_mults(pool, mults)
+    assert len(vals) == len(subs)
+    for k, sub in enumerate(subs):
+        ck = 0.3 - 0.2j * k
+        expect = sum(mults[l] * ck * c * oracle.expval(psi, n, [(1.0, term)]) for l, (c, term) in enumerate(sub))
+        assert abs(vals[k] - expect) < E_TOL, k
+    # a pool with a non-Pauli term: same formula through the gate path
+    gp = qf.QubitOpPool()
+    gen = qf.QubitOperator()
+    gen.add(0.5, qf.build_circuit("H_0"))
+    gen.add(-0.25j, qf.build_circuit("Z_1 X_2"))
+    gp.add(2.0, gen)
+    v = qc.direct_oppl_exp_val_w_mults(gp, [1.5, -0.5])
+    h0 = oracle.apply_gate(psi, n, oracle.G("H", 0))
+    expect = 2.0 * (1.5 * 0.5 * np.vdot(psi, h0) - 0.5 * -0.25j * oracle.expval(psi, n, [(1.0, [("Z", 1), ("X", 2)])]))
+    assert abs(v[0] - expect) < E_TOL
+    with pytest.raises(IndexError):
+        qc.direct_oppl_exp_val_w_mults(pool, mults[:2])
+
+
 @pytest.mark.parametrize("n", [4, 11, 17])
 def test_operator_application_and_expectations(qf, oracle, n):
     terms = W.random_pauli_sum(n, 60, seed=41, wmax=min(6, n), real_coeffs=False)

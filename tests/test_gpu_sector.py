@@ -310,3 +310,43 @@ def test_pair_words_and_list_form_agree_bit_for_bit(qf):
         vals = [eng.results[i] for i in ids]
         assert vals[0] == vals[1] == vals[2] == vals[3], norb
         assert list(eng.gradients[gids[0]]) == list(eng.gradients[gids[1]]), norb
+
+
+def test_captured_sequences_pass_the_host_audit(qf):
+    """Every node of a captured single-request sequence is checked on the host
+    before the graph is kept (one chain; the expected memcpy or kernel per
+    node; non-zero grids, blocks of 1..1024 threads; argument values and copy
+    extents equal to the addresses the template owns).  A full 18-qubit SD
+    template (the smallest size class on the captured grid-wide path) must
+    pass for the energy and the gradient sequence with every kernel's
+    arguments compared, and the replayed results must equal the full-space
+    program's."""
+    from qforte_amd.engine import workloads as W
+
+    n = 18
+    norb, na, nb, ref = W.half_filled_reference(n)
+    h, g = qf.synthetic_integrals(norb, 41)
+    hmap = qf.HamiltonianMap(norb)
+    inputs = hmap.pack(0.0, h, g)
+    pool = sd_pool(qf, ref)
+    tops = list(range(len(pool)))
+    tpl = qf.compile_ucc_template(n, ref, hmap, pool, tops)
+    for grad in (False, True):
+        nodes, kernels, kernels_checked, copies_checked = tpl.graph_audit(grad)
+        print(f"audit n={n} gradient={grad}: {nodes} nodes, {kernels} kernels, {kernels_checked} with arguments "
+              f"compared, {copies_checked} copies compared", flush=True)
+        copies = 3 if grad else 2  # request head up; energy (and generator overlaps) down
+        assert nodes == kernels + copies
+        assert kernels > len(tops)  # at least one launch per step
+        assert kernels_checked == kernels
+    t = list(np.random.default_rng(5).normal(0.0, 0.05, len(tops)))
+    prog = qf.compile_ucc_program(n, ref, hmap.operator(inputs), pool, tops)
+    e, gr = tpl.gradient(t, inputs)
+    e_p, g_p = prog.gradient(t)
+    assert abs(tpl.energy(t, inputs) - e_p) < E_TOL
+    assert abs(e - e_p) < E_TOL
+    assert np.max(np.abs(np.array(gr) - np.array(g_p))) < 1e-10
+    # a template whose sequence runs in one workgroup is never captured
+    _, _, ref8 = synthetic_case(qf, 4, 2, 2, 0)
+    small = qf.compile_ucc_template(8, ref8, qf.HamiltonianMap(4), sd_pool(qf, ref8), [0, 1])
+    assert small.graph_audit(True) == [0, 0, 0, 0]

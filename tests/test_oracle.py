@@ -222,6 +222,37 @@ def test_exponential_ladder_matches_reference(oracle, ref):
     assert np.max(np.abs(np.array(qc.get_coeff_vec()) - psi)) < 1e-13
 
 
+def test_ucc_state_matches_reference_and_expm(oracle, ref):
+    """ucc_state: |ref> (occupied spin-orbitals set, state_prep.py:5-11), then
+    exp(c_k P_k) in list order (trotterization.py:29-35).  Pinned two ways:
+    the compiled reference Computer running each exponential's gate ladder
+    from the determinant built by X gates, and a product of dense expm.  A
+    reversed product order or a wrong reference bit fails both."""
+    n = 6
+    occ = [1, 1, 0, 1, 0, 0]
+    strings = W.random_pauli_strings(n, 10, seed=23, wmin=1, wmax=4)
+    exps = [(-1j * theta, term) for theta, term in strings]
+    psi = oracle.ucc_state(n, occ, exps)
+    qc = ref.Computer(n)
+    for q, o in enumerate(occ):
+        if o:
+            qc.apply_gate(ref.gate("X", q, q))
+    for c, term in exps:
+        circ = ref.Circuit()
+        for g in oracle.exponentiate_gates(c, term):
+            circ.add(ref.gate(g.gtype, g.target, g.control, g.parameter) if g.gtype in ("Rx", "Ry", "Rz")
+                     else ref.gate(g.gtype, g.target, g.control))
+        qc.apply_circuit(circ)
+    assert np.max(np.abs(np.array(qc.get_coeff_vec()) - psi)) < 1e-13
+    dense = np.zeros(1 << n, complex)
+    dense[0b001011] = 1.0
+    for c, term in exps:
+        dense = scipy.linalg.expm(c * oracle.operator_matrix(n, [(1.0, term)])) @ dense
+    assert np.max(np.abs(dense - psi)) < 1e-13
+    backwards = oracle.ucc_state(n, occ, exps[::-1])
+    assert np.max(np.abs(backwards - psi)) > 1e-3
+
+
 def test_sa_sd_reference_recursion_is_not_the_derivative(oracle, qf):
     # uccvqeabc.py:151-244 applies the spin-adapted halves of operator mu + 1
     # in forward order to undo them and measures K_mu after all halves of mu.
