@@ -162,6 +162,15 @@ int qfe_apply_ops(qfe_state s, const qfe_op* ops, uint64_t count);
 /* ---- Pauli sums (computer.cc:80-91, 753-806) ----------------------------- */
 /* direct_op_exp_val for an operator of Pauli terms: <psi| sum_k coeff_k P_k |psi>. */
 int qfe_expval(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out);
+/* qfe_expval with no allocation: no reference counterpart; for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, hosts that forbid hipMalloc).  The group and term tables, partial
+ * sums and per-group results live in one fixed 6 MiB device array per GPU
+ * (resolved once by qfe_init), so at most 65536 terms per call (QFE_ERR_ARG
+ * beyond).  Result equal to qfe_expval's.  Calls that overlap on one GPU
+ * race on that array; every call synchronizes the library stream before it
+ * returns. */
+int qfe_expval_noalloc(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out);
 /* Several operators at once (direct_oppl_exp_val, computer.cc:773-806):
  * op_offsets[k]..op_offsets[k+1] delimit operator k; out[k] = <psi|O_k|psi>. */
 int qfe_expval_many(qfe_state s, const qfe_pauli* terms, const uint64_t* op_offsets, uint64_t nops,

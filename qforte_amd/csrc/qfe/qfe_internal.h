@@ -178,6 +178,9 @@ struct EvPass {
 };
 void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes);
 
+// device address of the fixed expectation scratch on `device` (current)
+int noalloc_scratch_setup(int device);
+
 // device helpers exposed across translation units
 int launch_zero(double2* d, uint64_t N, hipStream_t st);
 int launch_basis(double2* d, uint64_t N, uint64_t idx, hipStream_t st);

@@ -348,32 +348,29 @@ static void launch_orbit_expval_k(const double2* bra, const double2* ket, uint64
                            b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
 }
 
-static int expval_groups_orbit(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
-                               const std::vector<uint64_t>& zs, const std::vector<double2>& cs,
-                               double2* out_per_group) {
-    Context& cx = ctx();
-    const size_t G = groups.size();
-    const uint64_t N = 1ull << n;
+// One orbit expectation pass as planned on the host: its basis (pivots
+// ascending) and its run of OrbitGroup entries.
+struct OrbitLaunch {
+    int M;
+    uint64_t b[4];
+    uint32_t piv[4];
+    size_t g0, ng;
+};
+
+// Host tables of every pass: per pass its OrbitGroup run (slot = group index)
+// and OrbitTerm run, in pass order.
+static void build_orbit_tables(const std::vector<PauliGroup>& groups, const std::vector<uint64_t>& zs,
+                               const std::vector<double2>& cs, std::vector<OrbitGroup>& og, std::vector<OrbitTerm>& ot,
+                               std::vector<OrbitLaunch>& launches) {
     std::vector<EvPass> passes;
     plan_expval(groups, passes);
-    uint64_t nb64 = N / 4096;
-    if (nb64 < 1) nb64 = 1;
-    if (nb64 > 2048) nb64 = 2048;
-    const uint32_t nb = (uint32_t)nb64;
-    // host tables: per pass its OrbitGroup run and OrbitTerm run
-    std::vector<OrbitGroup> og;
-    std::vector<OrbitTerm> ot;
-    struct Launch {
-        int M;
-        uint64_t b[4];
-        uint32_t piv[4];
-        size_t g0, ng;
-    };
-    std::vector<Launch> launches;
+    og.clear();
+    ot.clear();
+    launches.clear();
     for (const EvPass& p : passes) {
         std::vector<uint64_t> b = p.basis;
         std::sort(b.begin(), b.end(), [](uint64_t u, uint64_t v) { return __builtin_clzll(u) > __builtin_clzll(v); });
-        Launch L{};
+        OrbitLaunch L{};
         L.M = (int)b.size();
         for (int k = 0; k < L.M; ++k) {
             L.b[k] = b[k];
@@ -410,6 +407,49 @@ static int expval_groups_orbit(const double2* bra, const double2* ket, int n, co
         }
         launches.push_back(L);
     }
+}
+
+// Workgroups of an orbit expectation pass over 2^n amplitudes.
+static uint32_t orbit_expval_blocks(int n) {
+    uint64_t nb64 = (1ull << n) / 4096;
+    if (nb64 < 1) nb64 = 1;
+    if (nb64 > 2048) nb64 = 2048;
+    return (uint32_t)nb64;
+}
+
+static int launch_orbit_pass(const double2* bra, const double2* ket, int n, const OrbitLaunch& L,
+                             const OrbitGroup* dg, const OrbitTerm* dt, double2* partial, uint32_t nb) {
+    Context& cx = ctx();
+    const uint64_t N = 1ull << n;
+    const uint64_t norbit = N >> L.M;
+    const OrbitGroup* g = dg + L.g0;
+    const uint32_t ng = (uint32_t)L.ng;
+    const bool nt = use_nt(n, L.M == 0 ? 63u : L.piv[0]);  // pivots ascending
+    switch (L.M) {
+        case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+        case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+        case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+        case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+        case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+        default: return fail(QFE_ERR_STATE, "internal: expectation orbit dimension out of range");
+    }
+    QFE_HIP(hipGetLastError());
+    cx.counters.reduce++;
+    cx.counters.nt += nt ? 1 : 0;
+    cx.counters.bytes += (bra == ket ? 16.0 : 32.0) * (double)N;
+    return QFE_OK;
+}
+
+static int expval_groups_orbit(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
+                               const std::vector<uint64_t>& zs, const std::vector<double2>& cs,
+                               double2* out_per_group) {
+    Context& cx = ctx();
+    const size_t G = groups.size();
+    const uint32_t nb = orbit_expval_blocks(n);
+    std::vector<OrbitGroup> og;
+    std::vector<OrbitTerm> ot;
+    std::vector<OrbitLaunch> launches;
+    build_orbit_tables(groups, zs, cs, og, ot, launches);
     const size_t b_g = align256(og.size() * sizeof(OrbitGroup));
     const size_t b_t = align256(ot.size() * sizeof(OrbitTerm));
     const size_t b_part = align256(G * (size_t)nb * sizeof(double2));
@@ -429,23 +469,9 @@ static int expval_groups_orbit(const double2* bra, const double2* ket, int n, co
     const OrbitTerm* dt = reinterpret_cast<const OrbitTerm*>(base + b_g);
     double2* partial = reinterpret_cast<double2*>(base + b_g + b_t);
     double2* dout = reinterpret_cast<double2*>(base + b_g + b_t + b_part);
-    for (const Launch& L : launches) {
-        const uint64_t norbit = N >> L.M;
-        const OrbitGroup* g = dg + L.g0;
-        const uint32_t ng = (uint32_t)L.ng;
-        const bool nt = use_nt(n, L.M == 0 ? 63u : L.piv[0]);  // pivots ascending
-        switch (L.M) {
-            case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-            case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-            case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-            case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-            case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-            default: return fail(QFE_ERR_STATE, "internal: expectation orbit dimension out of range");
-        }
-        QFE_HIP(hipGetLastError());
-        cx.counters.reduce++;
-        cx.counters.nt += nt ? 1 : 0;
-        cx.counters.bytes += (bra == ket ? 16.0 : 32.0) * (double)N;
+    for (const OrbitLaunch& L : launches) {
+        rc = launch_orbit_pass(bra, ket, n, L, dg, dt, partial, nb);
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)G), dim3(kEvTPB), 0, cx.stream, partial, nb, dout);
     QFE_HIP(hipGetLastError());
@@ -488,6 +514,128 @@ int expval_groups(const double2* bra, const double2* ket, int n, const std::vect
     cx.counters.reduce += G;
     const double per = (bra == ket ? 16.0 : 32.0) * (double)(1ull << n);
     cx.counters.bytes += per * (double)G;
+    return QFE_OK;
+}
+
+// ---- expectation values with no allocation ------------------------------------------------
+// qfe_expval_noalloc keeps everything beyond its input state and output in
+// one fixed device array per GPU: the group and term tables, the per-block
+// partial sums and the per-group results.  Orbit passes run in windows of
+// whole passes whose partial sums fit the array; each window's groups are
+// summed (k_sum_groups, the same fixed order) before the next window reuses
+// the partials, so every group's value, and the host sum over groups in group
+// order, is the one qfe_expval computes.  Calls that overlap on one GPU race
+// on the array.
+constexpr uint64_t kNoAllocMaxTerms = 1ull << 16;
+constexpr size_t kNoAllocBytes = 6ull << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static void* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device (called by qfe_init with
+// `device` current, once per device).
+int noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return fail(QFE_ERR_ARG, "qfe_init: device index beyond the scratch table");
+    if (g_scratch_dev[device]) return QFE_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(expectation scratch)");
+    if (!ptr) return fail(QFE_ERR_DEVICE, "hipGetSymbolAddress(expectation scratch): null address");
+    g_scratch_dev[device] = ptr;
+    return QFE_OK;
+}
+
+// Bump layout inside the fixed array; false when the request does not fit.
+struct FixedLayout {
+    char* base;
+    size_t cap, used = 0;
+    char* take(size_t bytes) {
+        bytes = align256(bytes);
+        if (bytes > cap - used) return nullptr;
+        char* p = base + used;
+        used += bytes;
+        return p;
+    }
+};
+
+static int expval_groups_fixed(const double2* bra, const double2* ket, int n, const std::vector<PauliGroup>& groups,
+                               const std::vector<uint64_t>& zs, const std::vector<double2>& cs, double2* out_per_group,
+                               void* dev_scratch) {
+    Context& cx = ctx();
+    const size_t G = groups.size();
+    if (G == 0) return QFE_OK;
+    FixedLayout A{static_cast<char*>(dev_scratch), kNoAllocBytes};
+    std::vector<double2> res(G);
+    if (n < kEvOrbitMax + 8) {
+        const uint32_t nchunk = chunks_for(n, G);
+        char* dg = A.take(G * sizeof(PauliGroup));
+        char* dz = A.take(zs.size() * sizeof(uint64_t));
+        char* dc = A.take(cs.size() * sizeof(double2));
+        char* part = A.take(G * nchunk * sizeof(double2));
+        char* dout = A.take(G * sizeof(double2));
+        if (!dg || !dz || !dc || !part || !dout) return fail(QFE_ERR_ARG, "qfe_expval_noalloc: operator exceeds the fixed scratch");
+        QFE_HIP(hipMemcpyAsync(dg, groups.data(), G * sizeof(PauliGroup), hipMemcpyHostToDevice, cx.stream));
+        QFE_HIP(hipMemcpyAsync(dz, zs.data(), zs.size() * sizeof(uint64_t), hipMemcpyHostToDevice, cx.stream));
+        QFE_HIP(hipMemcpyAsync(dc, cs.data(), cs.size() * sizeof(double2), hipMemcpyHostToDevice, cx.stream));
+        hipLaunchKernelGGL(k_expval_groups, dim3((unsigned)((uint64_t)G * nchunk)), dim3(kEvTPB), 0, cx.stream, bra, ket,
+                           n, (const PauliGroup*)dg, (const uint64_t*)dz, (const double2*)dc, nchunk, (double2*)part);
+        QFE_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)G), dim3(kEvTPB), 0, cx.stream, (const double2*)part, nchunk,
+                           (double2*)dout);
+        QFE_HIP(hipGetLastError());
+        QFE_HIP(hipMemcpyAsync(res.data(), dout, G * sizeof(double2), hipMemcpyDeviceToHost, cx.stream));
+        QFE_HIP(hipStreamSynchronize(cx.stream));
+        std::memcpy(out_per_group, res.data(), G * sizeof(double2));
+        cx.counters.reduce += G;
+        cx.counters.bytes += (bra == ket ? 16.0 : 32.0) * (double)(1ull << n) * (double)G;
+        return QFE_OK;
+    }
+    const uint32_t nb = orbit_expval_blocks(n);
+    std::vector<OrbitGroup> og;
+    std::vector<OrbitTerm> ot;
+    std::vector<OrbitLaunch> launches;
+    build_orbit_tables(groups, zs, cs, og, ot, launches);
+    char* dg = A.take(og.size() * sizeof(OrbitGroup));
+    char* dt = A.take(ot.size() * sizeof(OrbitTerm));
+    char* dout = A.take(G * sizeof(double2));
+    const size_t row = (size_t)nb * sizeof(double2);
+    const size_t window = (A.cap - A.used) / row;  // groups whose partials fit at once
+    char* part = A.take(window * row);
+    if (!dg || !dt || !dout || !part || window < (size_t)kEvOrbitGroups)
+        return fail(QFE_ERR_ARG, "qfe_expval_noalloc: operator exceeds the fixed scratch");
+    // window-local output rows: whole passes in order, slot = position in the
+    // window (build_orbit_tables left the group index in slot)
+    std::vector<uint32_t> gid(og.size());
+    for (size_t i = 0; i < og.size(); ++i) gid[i] = og[i].slot;
+    std::vector<size_t> win_start;  // first launch of each window
+    size_t w0 = 0;
+    for (size_t l = 0; l < launches.size(); ++l) {
+        if (l == 0 || launches[l].g0 + launches[l].ng - w0 > window) {
+            win_start.push_back(l);
+            w0 = launches[l].g0;
+        }
+        for (size_t i = 0; i < launches[l].ng; ++i) og[launches[l].g0 + i].slot = (uint32_t)(launches[l].g0 + i - w0);
+    }
+    win_start.push_back(launches.size());
+    QFE_HIP(hipMemcpyAsync(dg, og.data(), og.size() * sizeof(OrbitGroup), hipMemcpyHostToDevice, cx.stream));
+    QFE_HIP(hipMemcpyAsync(dt, ot.data(), ot.size() * sizeof(OrbitTerm), hipMemcpyHostToDevice, cx.stream));
+    for (size_t w = 0; w + 1 < win_start.size(); ++w) {
+        const size_t l0 = win_start[w], l1 = win_start[w + 1];
+        const size_t g0 = launches[l0].g0, g1 = launches[l1 - 1].g0 + launches[l1 - 1].ng;
+        for (size_t l = l0; l < l1; ++l) {
+            int rc = launch_orbit_pass(bra, ket, n, launches[l], (const OrbitGroup*)dg, (const OrbitTerm*)dt,
+                                       (double2*)part, nb);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)(g1 - g0)), dim3(kEvTPB), 0, cx.stream, (const double2*)part,
+                           nb, (double2*)dout + g0);
+        QFE_HIP(hipGetLastError());
+    }
+    // rows of dout are in pass order
+    QFE_HIP(hipMemcpyAsync(res.data(), dout, G * sizeof(double2), hipMemcpyDeviceToHost, cx.stream));
+    QFE_HIP(hipStreamSynchronize(cx.stream));
+    for (size_t i = 0; i < og.size(); ++i) out_per_group[gid[i]] = res[i];
     return QFE_OK;
 }
 
@@ -551,6 +699,28 @@ int qfe_expval(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out
     group_terms(terms, count, groups, zs, cs);
     std::vector<double2> per(groups.size());
     rc = expval_groups(s->d, s->d, s->n, groups, zs, cs, per.data());
+    if (rc) return rc;
+    double2 tot = make_double2(0.0, 0.0);
+    for (const double2& v : per) tot = make_double2(tot.x + v.x, tot.y + v.y);
+    out->re = tot.x;
+    out->im = tot.y;
+    return QFE_OK;
+}
+
+int qfe_expval_noalloc(qfe_state s, const qfe_pauli* terms, uint64_t count, qfe_c64* out) {
+    if (!valid(s) || !out) return fail(QFE_ERR_ARG, "qfe_expval_noalloc: invalid argument");
+    if (count > kNoAllocMaxTerms)
+        return fail(QFE_ERR_ARG, "qfe_expval_noalloc: at most 65536 Pauli terms per call (fixed 6 MiB device scratch)");
+    int rc = check_terms(terms, count, s->n);
+    if (rc) return rc;
+    void* dev = g_scratch_dev[ctx().device];
+    if (!dev) return fail(QFE_ERR_STATE, "qfe_expval_noalloc: library not initialised on this device");
+    std::vector<PauliGroup> groups;
+    std::vector<uint64_t> zs;
+    std::vector<double2> cs;
+    group_terms(terms, count, groups, zs, cs);
+    std::vector<double2> per(groups.size());
+    rc = expval_groups_fixed(s->d, s->d, s->n, groups, zs, cs, per.data(), dev);
     if (rc) return rc;
     double2 tot = make_double2(0.0, 0.0);
     for (const double2& v : per) tot = make_double2(tot.x + v.x, tot.y + v.y);

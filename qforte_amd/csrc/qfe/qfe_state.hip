@@ -341,6 +341,8 @@ int qfe_init(int device) {
     if (device < 0 || device >= n) return fail(QFE_ERR_ARG, "qfe_init: device index out of range");
     QFE_HIP(hipSetDevice(device));
     QFE_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    int rc = noalloc_scratch_setup(device);
+    if (rc) return rc;
     g_ctx.device = device;
     g_ctx.ready = true;
     return QFE_OK;
