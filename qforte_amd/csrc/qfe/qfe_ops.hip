@@ -346,6 +346,11 @@ static int launch_single(qfe_state_s* st, const HostOp& h, double2* dev_mat) {
 // ---------------------------------------------------------------------------
 
 constexpr int kFusedTPB = 256;
+// DevOp::pad of the head op of a group of one-qubit ops inside a tile pass:
+// the group length, and kGroupPlain when every op of the group is a dense
+// U1 with no condition.
+constexpr uint32_t kGroupLen = 0xffu;
+constexpr uint32_t kGroupPlain = 0x100u;
 constexpr int kMaxTileQubits = 13;  // 2^13 * 16 B = 128 KiB of LDS
 
 struct PassDesc {
@@ -469,6 +474,53 @@ __device__ __forceinline__ void fused_group(double2* tile, const TileMap<KQ>& P,
     }
 }
 
+// Dense U1 along item axis AX: every pair of the 8 amplitudes.
+template <int AX>
+__device__ __forceinline__ void plain_axis(double2 (&a)[8], const DevOp& o) {
+    const double2 m0 = o.m[0], m1 = o.m[1], m2 = o.m[2], m3 = o.m[3];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i & (1 << AX)) continue;
+        const int j = i | (1 << AX);
+        const double2 x = a[i], y = a[j];
+        a[i] = cmix(m0, x, m1, y);
+        a[j] = cmix(m2, x, m3, y);
+    }
+}
+
+// A group of 2 or 3 dense U1 ops with no condition at all (kGroupPlain):
+// fused_group without the per-element condition tests and swap selects, as
+// straight-line code (no loop-carried amplitude array, so no register copies).
+template <uint32_t T, int KQ>
+__device__ __forceinline__ void plain_group(double2* tile, const TileMap<KQ>& P, const DevOp* __restrict__ g,
+                                            uint32_t cnt) {
+    const uint32_t t0 = g[0].t, t1 = g[1].t;
+    uint32_t t2 = 0;
+    if (cnt > 2) {
+        t2 = g[2].t;
+    } else {
+        while (t2 == t0 || t2 == t1) ++t2;
+    }
+    const uint32_t lo = min(t0, min(t1, t2)), hi = max(t0, max(t1, t2));
+    const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
+    const uint32_t p0 = P.lin(1u << t0), p1 = P.lin(1u << t1), p2 = P.lin(1u << t2);
+    for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
+        const uint32_t pb = P(ins0_32(ins0_32(ins0_32(it, lo), mid), hi));
+        uint32_t pe[8];
+        double2 a[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            pe[i] = pb ^ ((i & 1) ? p0 : 0u) ^ ((i & 2) ? p1 : 0u) ^ ((i & 4) ? p2 : 0u);
+            a[i] = tile[pe[i]];
+        }
+        plain_axis<0>(a, g[0]);
+        plain_axis<1>(a, g[1]);
+        if (cnt > 2) plain_axis<2>(a, g[2]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) tile[pe[i]] = a[i];
+    }
+}
+
 template <int KQ>
 __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <= 11 ? 4 : 1))) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
                                                      uint32_t nt, const DevOp* __restrict__ ops, uint32_t nops) {
@@ -508,11 +560,14 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
     for (uint32_t oi = 0; oi < nops; ++oi) {
         __syncthreads();
         const DevOp& o = ops[oi];
-        if (o.kind == K_U1 && o.pad >= 2) {
-            // a group of o.pad one-qubit ops on distinct tile bits: one LDS
-            // round trip for the whole group
-            const uint32_t cnt = o.pad;
-            fused_group<T, KQ>(tile, P, base, ops + oi, cnt);
+        if (o.kind == K_U1 && (o.pad & kGroupLen) >= 2) {
+            // a group of one-qubit ops on distinct tile bits: one LDS round
+            // trip for the whole group
+            const uint32_t cnt = o.pad & kGroupLen;
+            if (o.pad & kGroupPlain)
+                plain_group<T, KQ>(tile, P, ops + oi, cnt);
+            else
+                fused_group<T, KQ>(tile, P, base, ops + oi, cnt);
             oi += cnt - 1;
             continue;
         }
@@ -1254,7 +1309,8 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
 }
 
 // Mark runs of 2 or 3 consecutive U1 ops with distinct tile targets (the head
-// op's pad holds the run length): the fused kernel applies each run with one
+// op's pad holds the run length, with kGroupPlain when every op of the run is
+// a dense U1 with no condition): the fused kernel applies each run with one
 // LDS round trip.
 static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count, int k) {
     if (k < 3) return;
@@ -1276,6 +1332,12 @@ static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count
         }
         for (uint32_t r = 1; r < len; ++r) dops[first + j + r].pad = 0;
         h.pad = len >= 2 ? len : 0;
+        bool plain = len >= 2;
+        for (uint32_t r = 0; r < len; ++r) {
+            const DevOp& o = dops[first + j + r];
+            plain = plain && !o.hb && o.cmask == 0 && o.omask == 0;
+        }
+        if (plain) h.pad |= kGroupPlain;
         j += len;
     }
 }

@@ -228,6 +228,28 @@ struct OrbitTerm {
 };
 
 
+// p_i = conj(b_i) a_{i ^ v} for every element of an orbit, v dispatched to a
+// compile-time constant so that a_{i ^ v} names a register (no data movement).
+template <int M, uint32_t V = 0>
+__device__ __forceinline__ void partner_products(uint32_t v, double2 (&p)[1 << M], const double2 (&a)[1 << M],
+                                                 const double2 (&b)[1 << M]) {
+    if constexpr (V + 1 < (1u << M)) {
+        if (v != V) {
+            partner_products<M, V + 1>(v, p, a, b);
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < (1 << M); ++i) {
+        // an empty volatile asm on the partner keeps the products inside this
+        // branch: otherwise the compiler computes the products of every
+        // branch up front and spills them
+        double2 t = a[i ^ V];
+        asm volatile("" : "+v"(t.x), "+v"(t.y));
+        p[i] = cconjmul(b[i], t);
+    }
+}
+
 // SAME: bra and ket are one state (plain expectation value): load it once.
 template <int M, bool SAME, bool NT>
 __global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(SAME ? 3 : 2))) void k_orbit_expval(const double2* __restrict__ bra,
@@ -269,29 +291,24 @@ __global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(SAME ? 3
         for (uint32_t g = 0; g < ngroups; ++g) {
             const OrbitGroup G = groups[g];
             double2 acc = make_double2(0.0, 0.0);
-            // p_i = conj(bra_i) ket_{i ^ v}
+            // p_i = conj(bra_i) ket_{i ^ v}: the group's flip mask selects, at
+            // compile time, which register holds the partner of each element
+            // (no data movement)
             double2 p[1 << M];
-#pragma unroll
-            for (int i = 0; i < (1 << M); ++i) p[i] = a[i];
-            xor_permute<M>(p, G.v);
-#pragma unroll
-            for (int i = 0; i < (1 << M); ++i) p[i] = cconjmul(b[i], p[i]);
+            partner_products<M>(G.v, p, a, b);
             for (uint32_t tt = 0; tt < G.count; ++tt) {
                 const OrbitTerm T = terms[G.first + tt];
-                const uint32_t flip = (__popcll(j & T.z) & 1) ? ~0u : 0u;
-                // bit i of pmask: the sign of the ket element i ^ v
-                const uint32_t sm = T.pmask ^ flip;
+                // bit i of pmask: the sign of the ket element i ^ v; the signs
+                // are uniform, so each element is one FMA by +-1, and the
+                // thread's own parity negates the sum
                 double2 w = make_double2(0.0, 0.0);
 #pragma unroll
                 for (int i = 0; i < (1 << M); ++i) {
-                    if ((sm >> i) & 1u) {
-                        w.x -= p[i].x;
-                        w.y -= p[i].y;
-                    } else {
-                        w.x += p[i].x;
-                        w.y += p[i].y;
-                    }
+                    const double sg = ((T.pmask >> i) & 1u) ? -1.0 : 1.0;
+                    w.x = fma(sg, p[i].x, w.x);
+                    w.y = fma(sg, p[i].y, w.y);
                 }
+                if (__popcll(j & T.z) & 1) w = make_double2(-w.x, -w.y);
                 acc = cadd(acc, cmul(T.c, w));
             }
             acc_s[g * kEvTPB + threadIdx.x] = cadd(acc_s[g * kEvTPB + threadIdx.x], acc);
