@@ -168,6 +168,17 @@ def run_open_loop(qf, classes, reqs, rate, seed):
     return makespan, lat, [eng.results[r] for r in ids]
 
 
+def latency_by_class(reqs, lat):
+    """p50 / p99 latency (ms) of each size class of an open-loop run."""
+    import numpy as np
+
+    out = {}
+    for n in sorted({r[0] for r in reqs}):
+        v = np.array([lat[i] for i, r in enumerate(reqs) if r[0] == n])
+        out[n] = {"p50_ms": round(1e3 * float(np.percentile(v, 50)), 3), "p99_ms": round(1e3 * float(np.percentile(v, 99)), 3)}
+    return out
+
+
 def serving_leg(qf, W, nreq, plain_per_class):
     """Continuous-batching UCC energy serving: capacity (all requests queued
     at once), three Poisson rates relative to it, and the plain
@@ -203,6 +214,7 @@ def serving_leg(qf, W, nreq, plain_per_class):
             "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
             "max_abs_diff_vs_capacity_run": same,
+            "by_class": latency_by_class(reqs, lat),
         }
         print(f"[bench] serving at {f}x: {rates[f'{f}x']}", file=sys.stderr, flush=True)
 
@@ -410,6 +422,31 @@ def gradient_leg(qf, W, nqubit=20, seed=20, fd_components=6, full_pool_sample=12
             "jw_compile_s": round(jw_s, 2),
         },
     }
+
+
+def host_description():
+    """The host the CPU baseline ran on: cores this process may use, CPU model
+    and RAM (the baseline itself runs on one of those cores)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ram = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    ram = round(int(line.split()[1]) / 2**20, 1)
+                    break
+    except OSError:
+        pass
+    return {"affinity_cores": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count(), "cpu_model": model,
+            "ram_gib": ram, "threads_used": 1}
 
 
 def cpu_baseline(W, n_cpu, oracle_ref_loader, qf=None):
@@ -621,6 +658,7 @@ def main():
                            f"qubits: 1 brickwork layer ({res['gates']} gates) + {res['exps']} Pauli exponentials "
                            f"(gate ladders) + {res['terms']}-term expectation, {res['seconds']:.2f} s"),
                 "seconds": round(res["seconds"], 3),
+                "host": host_description(),
                 "gpu_effective_GBps_same_unit": round(eff_gbps, 1),
                 "parity_vs_gpu": res["parity"],
             }

@@ -218,16 +218,16 @@ int qfe_program_energy_graph(qfe_program p, const double* params, double* energy
  * forward sweep, sigma = H psi, then a reverse sweep un-applying each
  * exponential on psi and sigma.  grad has nparam entries; energy may be NULL. */
 int qfe_program_gradient(qfe_program p, const double* params, double* grad, double* energy);
-/* ADAPT pool gradients 2 Re <H psi | A_k psi> for many operators A_k at the
- * state U(t)|ref> (measure_gradient3, uccvqeabc.py:253-283).  Operators are
- * delimited by op_offsets; processed in batches of up to `batch` operators
- * per launch. */
 /* PQE residual amplitudes (UCCNPQE.get_residual_vector, uccnpqe.py:249-292:
  * apply_circuit(U), apply_operator(H), apply_circuit(U.adjoint()), then
  * coeff at each excited determinant): out[i] = <idx[i]| U(t)^dag H U(t) |ref>,
  * idx[i] a basis index of the register.  The caller multiplies by the
  * determinant's phase. */
 int qfe_program_residuals(qfe_program p, const double* params, const uint64_t* idx, uint64_t count, qfe_c64* out);
+/* ADAPT pool gradients 2 Re <H psi | A_k psi> for many operators A_k at the
+ * state U(t)|ref> (measure_gradient3, uccvqeabc.py:253-283).  Operators are
+ * delimited by op_offsets; processed in batches of up to `batch` operators
+ * per launch. */
 int qfe_program_pool_gradients(qfe_program p, const double* params, const qfe_pauli* pool_terms,
                                const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 
@@ -280,13 +280,13 @@ int qfe_template_prepare(qfe_template t, const double* params, qfe_state out);
  * sigma; grad[p] = sum over generators k with param p of 2 scale_k
  * Re<sigma_k|K_k|psi_k>.  energy (may be NULL) receives <psi|H|psi>. */
 int qfe_template_gradient(qfe_template t, const double* params, const double* inputs, double* grad, double* energy);
-/* ADAPT pool gradients 2 Re<H psi|A_k psi> (uccvqeabc.py:253-283) for nops
- * anti-Hermitian operators given as Pauli sums (pool_terms split by
- * op_offsets); `batch` operators per launch (0: 64). */
 /* PQE residual amplitudes on the sector (as qfe_program_residuals:
  * uccnpqe.py:249-292); every idx[i] must lie in the template's sector. */
 int qfe_template_residuals(qfe_template t, const double* params, const double* inputs, const uint64_t* idx,
                            uint64_t count, qfe_c64* out);
+/* ADAPT pool gradients 2 Re<H psi|A_k psi> (uccvqeabc.py:253-283) for nops
+ * anti-Hermitian operators given as Pauli sums (pool_terms split by
+ * op_offsets); `batch` operators per launch (0: 64). */
 int qfe_template_pool_gradients(qfe_template t, const double* params, const double* inputs, const qfe_pauli* pool_terms,
                                 const uint64_t* op_offsets, uint64_t nops, uint64_t batch, double* grads);
 

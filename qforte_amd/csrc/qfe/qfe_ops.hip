@@ -18,7 +18,7 @@
 #include <utility>
 #include <vector>
 
-#include "qfe_common.cuh"
+#include "qfe_device.h"
 #include "qfe_internal.h"
 
 namespace qfe {

@@ -12,7 +12,7 @@
 #include <cstring>
 #include <vector>
 
-#include "qfe_common.cuh"
+#include "qfe_device.h"
 #include "qfe_internal.h"
 
 namespace qfe {

@@ -21,7 +21,7 @@
 // rounding leaves u at or above a level's total, the last position with
 // positive mass at that level is taken.
 
-#include "qfe_common.cuh"
+#include "qfe_device.h"
 #include "qfe_internal.h"
 
 namespace qfe {

@@ -11,7 +11,7 @@
 #include <unordered_map>
 #include <vector>
 
-#include "qfe_common.cuh"
+#include "qfe_device.h"
 #include "qfe_internal.h"
 
 namespace qfe {
