@@ -394,18 +394,27 @@ __device__ __forceinline__ void group_axis(double2 (&a)[8], const uint32_t (&e)[
 // update the map (P' = P o G: a CNOT c -> t adds col[t] to col[c], an X on t
 // adds col[t] to off); every other op reads and writes tile[P(l)] for its
 // logical indices l, and the write-back reads tile[P(l)] for amplitude l.
+// P is evaluated per element from two LDS tables (index bits 0-5 and 6-12),
+// rebuilt once after each run of map updates: two reads and an XOR instead
+// of a GF(2) product over every tile bit.
 template <int KQ>
 struct TileMap {
     uint32_t col[KQ];
     uint32_t off;
-    bool perm;  // false: P is the identity (no translation)
-    __device__ __forceinline__ void init() {
+    bool perm;   // false: P is the identity (no translation)
+    bool dirty;  // the lookup tables are older than the map
+    const uint32_t* tlo;  // LDS: P on index bits 0-5, with off
+    const uint32_t* thi;  // LDS: M on index bits 6 and up
+    __device__ __forceinline__ void init(const uint32_t* lo, const uint32_t* hi) {
 #pragma unroll
         for (int b = 0; b < KQ; ++b) col[b] = 1u << b;
         off = 0;
         perm = false;
+        dirty = false;
+        tlo = lo;
+        thi = hi;
     }
-    // M v (linear part only)
+    // M v (linear part only), bit by bit (uniform v, or table builds)
     __device__ __forceinline__ uint32_t lin(uint32_t v) const {
         if (!perm) return v;
         uint32_t r = 0;
@@ -413,7 +422,8 @@ struct TileMap {
         for (int b = 0; b < KQ; ++b) r ^= ((v >> b) & 1u) ? col[b] : 0u;
         return r;
     }
-    __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return perm ? (lin(l) ^ off) : l; }
+    // P(l) = M l ^ off: linear over GF(2), so it splits into two lookups
+    __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return perm ? (tlo[l & 63u] ^ thi[l >> 6]) : l; }
     // column of tile bit t (t uniform): a compare chain keeps col[] in registers
     __device__ __forceinline__ uint32_t column(uint32_t t) const {
         uint32_t r = 0;
@@ -434,6 +444,16 @@ struct TileMap {
             off ^= ct;
         }
         perm = true;
+        dirty = true;
+    }
+    // Rebuild the lookup tables after map changes (uniform; every thread calls it).
+    __device__ __forceinline__ void refresh(uint32_t* lo, uint32_t* hi) {
+        if (!dirty) return;
+        const uint32_t t = threadIdx.x;
+        if (t < 64) lo[t] = lin(t) ^ off;
+        if (t < (1u << (KQ > 6 ? KQ - 6 : 0))) hi[t] = lin(t << 6);
+        __syncthreads();
+        dirty = false;
     }
 };
 
@@ -526,6 +546,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
                                                      uint32_t nt, const DevOp* __restrict__ ops, uint32_t nops) {
     extern __shared__ double2 tile[];
     __shared__ uint64_t hioff[1 << 9];
+    __shared__ uint32_t ptab_lo[64], ptab_hi[1 << (kMaxTileQubits - 6)];
     constexpr uint32_t T = 1u << KQ;
     const uint32_t nhi = KQ - nlow;
     const uint64_t himask = bmask & ~((1ull << nlow) - 1ull);
@@ -556,10 +577,17 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
         tile[l] = nt ? ld_nt(src) : *src;
     }
     TileMap<KQ> P;
-    P.init();
+    P.init(ptab_lo, ptab_hi);
     for (uint32_t oi = 0; oi < nops; ++oi) {
-        __syncthreads();
         const DevOp& o = ops[oi];
+        if (o.kind == K_U1 && o.hb && __popc((uint32_t)o.cmask) <= 1) {
+            // a permutation of the tile: the map absorbs it (registers only,
+            // so no barrier)
+            if ((base & o.omask) == o.oval) P.apply_swap(o.t, (uint32_t)o.cmask, (uint32_t)o.cval);
+            continue;
+        }
+        __syncthreads();
+        P.refresh(ptab_lo, ptab_hi);
         if (o.kind == K_U1 && (o.pad & kGroupLen) >= 2) {
             // a group of one-qubit ops on distinct tile bits: one LDS round
             // trip for the whole group
@@ -575,12 +603,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
         switch (o.kind) {
             case K_U1: {
                 const uint32_t tb = 1u << o.t, ptb = P.lin(tb);
-                if (o.hb) {  // X on the target
-                    if (__popc((uint32_t)o.cmask) <= 1) {
-                        // a permutation of the tile: the map absorbs it
-                        P.apply_swap(o.t, (uint32_t)o.cmask, (uint32_t)o.cval);
-                        break;
-                    }
+                if (o.hb) {  // X on the target with two or more conditions
                     for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
                         const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
@@ -692,6 +715,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
         }
     }
     __syncthreads();
+    P.refresh(ptab_lo, ptab_hi);
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
         double2* dst = s + (base | hioff[l >> nlow] | (l & lowmask));
         const double2 v = tile[P(l)];
@@ -1308,36 +1332,94 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
     ops.swap(out);
 }
 
-// Mark runs of 2 or 3 consecutive U1 ops with distinct tile targets (the head
-// op's pad holds the run length, with kGroupPlain when every op of the run is
-// a dense U1 with no condition): the fused kernel applies each run with one
-// LDS round trip.
-static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count, int k) {
+// A dense one-qubit gate with no condition (a plain group member).
+static inline bool plain_u1(const DevOp& o) { return o.kind == K_U1 && !o.hb && o.cmask == 0; }
+// An X with at most one condition bit: the tile's index map absorbs it
+// (cX, anti-controlled X), no amplitude moves.
+static inline bool map_op(const HostOp& h) {
+    return h.op.kind == K_U1 && h.op.hb && __builtin_popcountll(h.op.cmask) <= 1;
+}
+
+// Order the ops of a tile pass for the kernel: any order in which every op
+// follows each earlier op whose support it shares (ops with disjoint supports
+// commute), chosen greedily so that index-map ops come as early as they can
+// and dense unconditioned one-qubit gates on distinct qubits come in groups
+// of three (one LDS round trip each).  Returns the new order and the plain
+// groups as (position, length).
+static void schedule_tile_ops(const std::vector<HostOp>& ops, std::vector<int>& order,
+                              std::vector<std::pair<size_t, uint32_t>>& groups) {
+    std::vector<int> rem(order), out, ready, take, plain, other;
+    out.reserve(order.size());
+    groups.clear();
+    while (!rem.empty()) {
+        // ready: no earlier remaining op shares a qubit with it
+        ready.clear();
+        uint64_t seen = 0;
+        for (size_t i = 0; i < rem.size(); ++i) {
+            const uint64_t sup = ops[rem[i]].support;
+            if (!(sup & seen)) ready.push_back((int)i);
+            seen |= sup;
+        }
+        take.clear();
+        for (int i : ready)
+            if (map_op(ops[rem[i]])) take.push_back(i);
+        if (take.empty()) {
+            plain.clear();
+            other.clear();
+            for (int i : ready) (plain_u1(ops[rem[i]].op) ? plain : other).push_back(i);
+            if (plain.size() >= 3 || (other.empty() && plain.size() >= 2)) {
+                const size_t len = std::min<size_t>(3, plain.size());
+                groups.push_back({out.size(), (uint32_t)len});
+                take.assign(plain.begin(), plain.begin() + len);
+            } else {
+                take.push_back(other.empty() ? plain[0] : other[0]);
+            }
+        }
+        for (int i : take) out.push_back(rem[i]);
+        std::sort(take.begin(), take.end());
+        for (size_t k = take.size(); k-- > 0;) rem.erase(rem.begin() + take[k]);
+    }
+    order.swap(out);
+}
+
+// Mark the groups of a tile pass in its localized op array: the plain groups
+// the schedule formed (kGroupPlain), and runs of 2 or 3 other consecutive U1
+// ops with distinct tile targets (the head op's pad holds the run length);
+// the fused kernel applies each group with one LDS round trip.  Index-map
+// ops stay alone (absorbed without moving amplitudes).
+static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count, int k,
+                          const std::vector<std::pair<size_t, uint32_t>>& plain) {
+    for (uint32_t j = 0; j < count; ++j) dops[first + j].pad = 0;
     if (k < 3) return;
+    std::vector<char> in_plain(count, 0);
+    for (const auto& g : plain) {
+        dops[first + g.first].pad = g.second | kGroupPlain;
+        for (uint32_t r = 0; r < g.second; ++r) in_plain[g.first + r] = 1;
+    }
+    auto groupable = [&](uint32_t j) {
+        const DevOp& o = dops[first + j];
+        return !in_plain[j] && o.kind == K_U1 && !(o.hb && __builtin_popcountll(o.cmask) <= 1);
+    };
     uint32_t j = 0;
     while (j < count) {
-        DevOp& h = dops[first + j];
-        h.pad = 0;
-        if (h.kind != K_U1) {
+        if (!groupable(j)) {
             ++j;
             continue;
         }
         uint32_t len = 1;
-        uint32_t used = 1u << h.t;
-        while (len < 3 && j + len < count) {
+        uint32_t used = 1u << dops[first + j].t;
+        while (len < 3 && j + len < count && groupable(j + len)) {
             const DevOp& o = dops[first + j + len];
-            if (o.kind != K_U1 || ((used >> o.t) & 1u)) break;
+            if ((used >> o.t) & 1u) break;
             used |= 1u << o.t;
             ++len;
         }
-        for (uint32_t r = 1; r < len; ++r) dops[first + j + r].pad = 0;
-        h.pad = len >= 2 ? len : 0;
-        bool plain = len >= 2;
+        bool allp = len >= 2;
         for (uint32_t r = 0; r < len; ++r) {
             const DevOp& o = dops[first + j + r];
-            plain = plain && !o.hb && o.cmask == 0 && o.omask == 0;
+            allp = allp && !o.hb && o.cmask == 0 && o.omask == 0;
         }
-        if (plain) h.pad |= kGroupPlain;
+        dops[first + j].pad = len >= 2 ? (len | (allp ? kGroupPlain : 0u)) : 0u;
         j += len;
     }
 }
@@ -1442,11 +1524,14 @@ int plan_build(std::vector<HostOp>& ops_in, int n, bool device_angles, OpPlan** 
         pd.nrest = (uint32_t)ri;
         pd.nops = (uint32_t)p.ops.size();
         P->pass_op0[pi] = dops.size();
-        for (int oi : p.ops) {
+        std::vector<int> order(p.ops);
+        std::vector<std::pair<size_t, uint32_t>> plain;
+        if (pd.k >= 3) schedule_tile_ops(ops, order, plain);
+        for (int oi : order) {
             if (ops[oi].src >= 0) refs.push_back({0, dops.size(), ops[oi].src});
             dops.push_back(localize(ops[oi].op, b, local_of));
         }
-        group_u1_runs(dops, P->pass_op0[pi], pd.nops, (int)pd.k);
+        group_u1_runs(dops, P->pass_op0[pi], pd.nops, (int)pd.k, plain);
         P->pass_pd[pi] = P->pdescs.size();
         P->pdescs.push_back(pd);
     }
