@@ -414,14 +414,16 @@ struct TileMap {
         tlo = lo;
         thi = hi;
     }
-    // M v (linear part only), bit by bit (uniform v, or table builds)
-    __device__ __forceinline__ uint32_t lin(uint32_t v) const {
-        if (!perm) return v;
+    // M v from the register columns, bit by bit (table builds)
+    __device__ __forceinline__ uint32_t lin_cols(uint32_t v) const {
         uint32_t r = 0;
 #pragma unroll
         for (int b = 0; b < KQ; ++b) r ^= ((v >> b) & 1u) ? col[b] : 0u;
         return r;
     }
+    // M v = P(v) ^ P(0) from the lookup tables (after refresh): two uniform
+    // LDS reads instead of a scalar loop over every tile bit
+    __device__ __forceinline__ uint32_t lin(uint32_t v) const { return perm ? (tlo[v & 63u] ^ thi[v >> 6] ^ tlo[0]) : v; }
     // P(l) = M l ^ off: linear over GF(2), so it splits into two lookups
     __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return perm ? (tlo[l & 63u] ^ thi[l >> 6]) : l; }
     // column of tile bit t (t uniform): a compare chain keeps col[] in registers
@@ -450,8 +452,8 @@ struct TileMap {
     __device__ __forceinline__ void refresh(uint32_t* lo, uint32_t* hi) {
         if (!dirty) return;
         const uint32_t t = threadIdx.x;
-        if (t < 64) lo[t] = lin(t) ^ off;
-        if (t < (1u << (KQ > 6 ? KQ - 6 : 0))) hi[t] = lin(t << 6);
+        if (t < 64) lo[t] = lin_cols(t) ^ off;
+        if (t < (1u << (KQ > 6 ? KQ - 6 : 0))) hi[t] = lin_cols(t << 6);
         __syncthreads();
         dirty = false;
     }
