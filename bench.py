@@ -665,6 +665,9 @@ def main():
                            f"(gate ladders) + {res['terms']}-term expectation, {res['seconds']:.2f} s"),
                 "seconds": round(res["seconds"], 3),
                 "host": host_description(),
+                "why_one_thread": ("the reference's gate and exponential sweeps are serial loops; its one OpenMP "
+                                   "loop (expectation terms, computer.cc:895) compiles serially in its default "
+                                   "build (CMakeLists.txt:10 USE_OpenMP OFF, the OpenMP block commented out)"),
                 "gpu_effective_GBps_same_unit": round(eff_gbps, 1),
                 "parity_vs_gpu": res["parity"],
             }
