@@ -32,12 +32,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per k_orbit_lanes<4, 2> launch at 30 qubits from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (KiB per dispatch: 8388759 and 16786363), each
-# scaled by the same counter's reading on k_scale, a plain sweep that reads
-# and writes exactly 16 GiB (8388621 and 16778282: factors 2.0000 and
-# 0.99994): 33562824 KiB.
-TRAFFIC_PER_LAUNCH = {30: 33562824 * 1024}
+# HBM bytes per launch of the exponential-stage kernel, by (qubits, orbit
+# width), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+# each scaled by the same counter's reading on k_scale, a plain sweep that
+# reads and writes exactly 16 GiB (FETCH 8388621-8388622 KiB: factor 2.0000;
+# WRITE 16778250-16778282 KiB: factor 0.99994).
+#   k_orbit_lanes<4, 3>: FETCH 8388762, WRITE 16782743 -> 33559207 KiB
+#   k_orbit_lanes<4, 2>: FETCH 8388759, WRITE 16786363 -> 33562824 KiB
+TRAFFIC_PER_LAUNCH = {(30, 7): 33559207 * 1024, (30, 6): 33562824 * 1024}
 METRIC = "HBM GB/s (% peak) at 30 qubits; batched UCC energy evals/s + p99 vs plain"
 
 
@@ -625,15 +627,22 @@ def main():
         single_ms = time_single_sweep(qf, W, qc, n)
         gate_ms = time_single_gate_sweep(qf, qc, n)
         copy_ms = time_attainable_sweep(qf, qc)
+        dims = qf.get_pass_tuning()
+        if dims > 4:
+            kname = (f"k_orbit_lanes<4, {dims - 4}> ({dims}-dimension register orbits over {1 << (dims - 4)} lanes "
+                     f"of 16 amplitudes")
+        else:
+            kname = f"k_orbit<{dims}> ({dims}-dimension register orbits in one lane"
         roofline = {
-            "kernel": ("k_orbit_lanes<4, 2> (6-dimension register orbits over 4 lanes of 16 amplitudes; the exponential stage: %d passes "
-                       "for %d exponentials, 2S = %.2f GB per launch)" % (passes, len(strings), 2 * S / 1e9)),
+            "kernel": (kname + "; the exponential stage: %d passes for %d exponentials, 2S = %.2f GB per launch)"
+                       % (passes, len(strings), 2 * S / 1e9)),
+            "stage_ms": round(avg_ms * passes, 1),
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": TRAFFIC_PER_LAUNCH.get(n),
+            "traffic": TRAFFIC_PER_LAUNCH.get((n, dims)),
             "avg_launch_ms": round(avg_ms, 4),
             "single_exponential_sweep": {"kernel": "k_pexp", "avg_launch_ms": round(single_ms, 4),
                                          "achieved": round(2 * S / (single_ms * 1e-3) / 1e9, 1),

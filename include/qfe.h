@@ -86,8 +86,8 @@ int qfe_synchronize(void);
  * counterpart (autotuner hook). */
 int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass);
 int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass);
-/* Largest register orbit of the planner, 1..6 dimensions (5 and 6 spread one
- * orbit over 2 or 4 lanes of a wave); no reference counterpart (autotuner hook). */
+/* Largest register orbit of the planner, 1..7 dimensions (5, 6 and 7 spread one
+ * orbit over 2, 4 or 8 lanes of a wave); no reference counterpart (autotuner hook). */
 int qfe_set_pass_tuning(int orbit_max_dims);
 int qfe_get_pass_tuning(int* orbit_max_dims);
 /* Largest number of determinant pairs for which a sector template stores one

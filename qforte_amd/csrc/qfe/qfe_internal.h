@@ -27,7 +27,7 @@ struct Tuning {
     int tile_qubits = 11;  // k: qubits held by one fused tile (2^k amplitudes in LDS)
     int low_qubits = 4;    // L: lowest qubits always in the tile (contiguous 2^L runs)
     int max_ops = 24;      // ops per fused pass before LDS traffic dominates
-    int orbit_dims = 6;    // max dimensions of a register orbit (5, 6: spread over 2 or 4 lanes)
+    int orbit_dims = 7;    // max dimensions of a register orbit (5, 6, 7: spread over 2, 4 or 8 lanes)
     uint64_t sector_words = 1ull << 25;  // max pair words of one sector template (256 MB)
 };
 

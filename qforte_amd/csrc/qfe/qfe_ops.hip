@@ -788,6 +788,7 @@ static DevOp localize(const DevOp& g, uint64_t bmask, const int* local_of) {
 // ---------------------------------------------------------------------------
 
 constexpr int kOrbitMax = 4;
+constexpr int kLaneOrbitMax = 7;  // lane orbits: 16 amplitudes per lane over 2, 4 or 8 lanes
 constexpr int kOrbitTPB = 256;
 
 struct OrbitOp {
@@ -893,14 +894,40 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit(double2* __restrict__ s, ui
 struct OrbitOp2 {
     uint32_t v;
     uint32_t pad;
-    uint64_t smask;  // bit i = parity(c_i & z), i over 2^(4+Q) elements
+    uint64_t smask[2];  // bit i of the 128-bit word = parity(c_i & z), i over 2^(4+Q) elements
     uint64_t z;
     double c, sn;
     double2 ph;
 };
 
+// The E = 2^R sign bits of lane group g (elements g * E .. g * E + E - 1).
+template <int R, int Q>
+__device__ __forceinline__ uint32_t group_signs(const OrbitOp2& op, uint32_t g) {
+    constexpr uint32_t WMASK = (R == 5) ? 0xFFFFFFFFu : ((1u << (1 << R)) - 1u);
+    const uint32_t bit = g << R;
+    if constexpr ((1 << (R + Q)) <= 64) {
+        return (uint32_t)(op.smask[0] >> bit) & WMASK;
+    } else {
+        const uint64_t word = (bit & 64u) ? op.smask[1] : op.smask[0];
+        return (uint32_t)(word >> (bit & 63u)) & WMASK;
+    }
+}
+
 __device__ __forceinline__ double2 shfl_xor_d2(double2 v, int mask) {
     return make_double2(__shfl_xor(v.x, mask, kWave), __shfl_xor(v.y, mask, kWave));
+}
+
+// x with the signs of both components flipped when bit k of w is set (an
+// exact sign change on the high words).
+__device__ __forceinline__ double2 flip_by_bit(double2 x, uint32_t w, int k) {
+    const int m = (int)((w << (31 - k)) & 0x80000000u);
+    return make_double2(__hiloint2double(__double2hiint(x.x) ^ m, __double2loint(x.x)),
+                        __hiloint2double(__double2hiint(x.y) ^ m, __double2loint(x.y)));
+}
+
+// (gx + i gy) q: one multiply and one FMA per component.
+__device__ __forceinline__ double2 gmul(double gx, double gy, double2 q) {
+    return make_double2(fma(gx, q.x, -gy * q.y), fma(gx, q.y, gy * q.x));
 }
 
 template <int R, int Q, bool NT>
@@ -911,6 +938,7 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
     constexpr int M = R + Q;
     constexpr int E = 1 << R;              // amplitudes per lane
     constexpr uint32_t OPW = kWave >> Q;  // orbits per wave
+    static_assert(M <= 7, "the sign mask holds 128 elements");
     constexpr uint32_t WMASK = (E == 32) ? 0xFFFFFFFFu : ((1u << E) - 1u);
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t wave = ((uint64_t)blockIdx.x * kOrbitTPB + threadIdx.x) / kWave;
@@ -938,40 +966,48 @@ __global__ __launch_bounds__(kOrbitTPB) void k_orbit_lanes(double2* __restrict__
         off[i] = x;
         a[i] = live ? ld_s<NT>(s + off[i]) : make_double2(0.0, 0.0);
     }
-    const uint32_t hbase = h << R;
     for (uint32_t oi = 0; oi < nops; ++oi) {
         const OrbitOp2 op = ops[oi];
-        const uint64_t sm = op.smask ^ ((__popcll(j & op.z) & 1) ? ~0ull : 0ull);
-        const double2 iph = make_double2(-op.ph.y, op.ph.x);  // i * phase
+        const uint32_t flip = (__popcll(j & op.z) & 1) ? WMASK : 0u;
         if (op.v == 0) {
+            const uint32_t wo = group_signs<R, Q>(op, h) ^ flip;
             const double2 fp = make_double2(op.c - op.sn * op.ph.y, op.sn * op.ph.x);
             const double2 fm = make_double2(op.c + op.sn * op.ph.y, -op.sn * op.ph.x);
 #pragma unroll
-            for (int i = 0; i < E; ++i) a[i] = cmul(((sm >> (hbase + i)) & 1ull) ? fm : fp, a[i]);
+            for (int i = 0; i < E; ++i) a[i] = cmul(((wo >> i) & 1u) ? fm : fp, a[i]);
             continue;
         }
-        // the partner of element i = h * E + i' is (h ^ vq) * E + (i' ^ V), V = v mod E, vq = v / E; its
-        // sign bit in sm is bit i' ^ V of the E-bit word of group h ^ vq
+        // a_i' = cos a_i + (-1)^{s(partner)} g a_partner with g = sin * i * phase.  The partner of element
+        // i = h * E + i' is (h ^ vq) * E + (i' ^ V), V = v mod E, vq = v / E; its sign bit is bit i' ^ V of
+        // the E-bit word of group h ^ vq.  The sign goes onto the fresh product g a_partner.
         const uint32_t vq = op.v >> R;
-        const uint32_t w = (uint32_t)(sm >> (hbase ^ (vq << R))) & WMASK;
+        const uint32_t w = group_signs<R, Q>(op, h ^ vq) ^ flip;
         const int lm = (int)(vq * OPW);
-        const double c = op.c, s0 = op.sn;
+        const double c = op.c, gx = -op.sn * op.ph.y, gy = op.sn * op.ph.x;
         dispatch_mask<R>(op.v & (uint32_t)(E - 1), [&](auto vc) {
             constexpr int V = (int)decltype(vc)::value;
-            // partner of element i: element i ^ V of this lane (lm == 0) or of lane ^ lm
-            double2 p[E];
+            if (lm == 0) {
+                // in-lane partners (V != 0 here): each pair updated together from its old values
 #pragma unroll
-            for (int i = 0; i < E; ++i) p[i] = a[i ^ V];
-            if (lm) {
+                for (int i = 0; i < E; ++i) {
+                    if (i < (i ^ V)) {
+                        const double2 x = a[i], y = a[i ^ V];
+                        const double2 ty = flip_by_bit(gmul(gx, gy, y), w, i ^ V);
+                        const double2 tx = flip_by_bit(gmul(gx, gy, x), w, i);
+                        a[i] = make_double2(fma(c, x.x, ty.x), fma(c, x.y, ty.y));
+                        a[i ^ V] = make_double2(fma(c, y.x, tx.x), fma(c, y.y, tx.y));
+                    }
+                }
+            } else {
+                // partner of element i: element i ^ V of lane ^ lm
+                double2 p[E];
 #pragma unroll
-                for (int i = 0; i < E; ++i) p[i] = shfl_xor_d2(p[i], lm);
-            }
+                for (int i = 0; i < E; ++i) p[i] = shfl_xor_d2(a[i ^ V], lm);
 #pragma unroll
-            for (int i = 0; i < E; ++i) {
-                // a_i' = cos a_i + i sin ph (-1)^{s(partner)} a_partner
-                const double sn = ((w >> (i ^ V)) & 1u) ? -s0 : s0;
-                const double2 t = cmul(iph, p[i]);
-                a[i] = make_double2(c * a[i].x + sn * t.x, c * a[i].y + sn * t.y);
+                for (int i = 0; i < E; ++i) {
+                    const double2 t = flip_by_bit(gmul(gx, gy, p[i]), w, i ^ V);
+                    a[i] = make_double2(fma(c, a[i].x, t.x), fma(c, a[i].y, t.y));
+                }
             }
         });
     }
@@ -1016,7 +1052,8 @@ struct Pass {
 // supports are disjoint (then the two commute exactly).  Each pass is grown
 // two ways from the first pending op -- as an LDS tile (mixing qubits fit
 // 2^k amplitudes) and as a register orbit (a run of exponentials whose flip
-// masks span <= 4 dimensions) -- and the one covering more ops is kept.
+// masks span at most orbit_dims <= 7 dimensions) -- and the one covering more
+// ops is kept.
 static void grow_tile(const std::vector<HostOp>& ops, const std::vector<char>& done, size_t first, int K,
                       uint64_t lowmask, uint64_t all, int max_ops, Pass& p) {
     p = Pass{};
@@ -1079,7 +1116,7 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
         if (first >= ops.size()) break;
         Pass tile, orbit;
         grow_tile(ops, done, first, K, lowmask, all, tu.max_ops, tile);
-        grow_orbit(ops, done, first, all, tu.max_ops, orbit, P_ORBIT, std::max(1, std::min(tu.orbit_dims, kOrbitMax + 2)));
+        grow_orbit(ops, done, first, all, tu.max_ops, orbit, P_ORBIT, std::max(1, std::min(tu.orbit_dims, kLaneOrbitMax)));
         Pass p;
         if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
             p = std::move(orbit);
@@ -1166,10 +1203,10 @@ static void build_orbit(const Pass& p, const std::vector<HostOp>& ops, OrbitDesc
 }
 
 // Lane-orbit descriptor (basis sorted by pivot) and per-op data for an orbit
-// pass of 5 or 6 dimensions.
+// pass of 5, 6 or 7 dimensions.
 struct LaneDesc {
-    uint64_t b[kOrbitMax + 2];
-    uint32_t piv[kOrbitMax + 2];
+    uint64_t b[kLaneOrbitMax];
+    uint32_t piv[kLaneOrbitMax];
 };
 
 static void build_orbit_lanes(const Pass& p, const std::vector<HostOp>& ops, LaneDesc& d, std::vector<OrbitOp2>& out) {
@@ -1194,7 +1231,7 @@ static void build_orbit_lanes(const Pass& p, const std::vector<HostOp>& ops, Lan
         for (int k = 0; k < M; ++k)
             if ((g.x >> d.piv[k]) & 1ull) o.v |= 1u << k;
         for (int i = 0; i < (1 << M); ++i)
-            if (__builtin_popcountll(c[i] & g.z) & 1) o.smask |= 1ull << i;
+            if (__builtin_popcountll(c[i] & g.z) & 1) o.smask[i >> 6] |= 1ull << (i & 63);
         o.z = g.z;
         o.c = g.m[0].x;
         o.sn = g.m[0].y;
@@ -1232,6 +1269,10 @@ static int launch_orbit_lanes(qfe_state_s* st, int M, uint32_t low_pivot, const 
         launch_orbit_lanes_k<6, true>(st->d, norbit, b, piv, ops, nops, cx.stream);
     else if (M == 6)
         launch_orbit_lanes_k<6, false>(st->d, norbit, b, piv, ops, nops, cx.stream);
+    else if (M == 7 && nt)
+        launch_orbit_lanes_k<7, true>(st->d, norbit, b, piv, ops, nops, cx.stream);
+    else if (M == 7)
+        launch_orbit_lanes_k<7, false>(st->d, norbit, b, piv, ops, nops, cx.stream);
     else
         return fail(QFE_ERR_STATE, "internal: lane orbit dimension out of range");
     QFE_HIP(hipGetLastError());

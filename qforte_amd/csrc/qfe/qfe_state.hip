@@ -388,7 +388,7 @@ int qfe_set_sector_word_limit(uint64_t max_pair_words) {
 }
 
 int qfe_set_pass_tuning(int orbit_max_dims) {
-    if (orbit_max_dims < 1 || orbit_max_dims > 6) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: orbit_max_dims must be in [1, 6]");
+    if (orbit_max_dims < 1 || orbit_max_dims > 7) return fail(QFE_ERR_ARG, "qfe_set_pass_tuning: orbit_max_dims must be in [1, 7]");
     g_ctx.tuning.orbit_dims = orbit_max_dims;
     return QFE_OK;
 }

@@ -13,7 +13,7 @@ the gate circuits, each with the other knob at its current value.
 import json
 
 DEFAULT_TILES = ((10, 4, 16), (11, 4, 16), (11, 4, 24), (11, 4, 32), (12, 4, 32))
-DEFAULT_ORBITS = (4, 5, 6)
+DEFAULT_ORBITS = (4, 5, 6, 7)
 
 
 def _time_circuits(qf, qc, circuits, reps):

@@ -437,10 +437,12 @@ def test_large_state_norm_and_replica(qf, oracle, n):
         assert np.max(np.abs(amps - psi)) < AMP_TOL
 
 
-@pytest.mark.parametrize("tuning,orbit_dims", [((11, 4, 24), 4), ((11, 2, 64), 6), ((11, 6, 9), 5), ((11, 4, 24), 6)])
+@pytest.mark.parametrize("tuning,orbit_dims", [((11, 4, 24), 4), ((11, 2, 64), 6), ((11, 6, 9), 5), ((11, 4, 24), 6),
+                                               ((11, 4, 24), 7), ((11, 2, 64), 7)])
 def test_wide_exponential_runs(qf, oracle, tuning, orbit_dims):
-    # 18 qubits: runs of exponentials whose flip masks span 5 or 6 dimensions
-    # go to register orbits spread over 2 or 4 lanes; mixed with repeated
+    # 18 qubits: runs of exponentials whose flip masks span 5, 6 or 7
+    # dimensions go to register orbits spread over 2, 4 or 8 lanes (the 7-
+    # dimension sign masks span two 64-bit words); mixed with repeated
     # masks, masks that combine earlier ones, pure Z strings and masks
     # confined to the low bits
     n = 18

@@ -43,6 +43,7 @@ def restore_tuning(qf, gpu):
 
 # (orbit dimensions, (tile qubits, low qubits, ops per pass), counter that must be non-zero)
 CASES = [
+    (7, (11, 4, 24), "lane_orbit_passes"),  # k_orbit_lanes<4, 3, NT>, k_fused<11>
     (6, (11, 4, 24), "lane_orbit_passes"),  # k_orbit_lanes<4, 2, NT>, k_fused<11>
     (5, (13, 4, 64), "lane_orbit_passes"),  # k_orbit_lanes<4, 1, NT>, k_fused<13>
     (4, (11, 4, 24), "orbit_passes"),       # k_orbit<M <= 4, NT>
