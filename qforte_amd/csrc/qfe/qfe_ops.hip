@@ -351,6 +351,11 @@ constexpr int kFusedTPB = 256;
 // U1 with no condition.
 constexpr uint32_t kGroupLen = 0xffu;
 constexpr uint32_t kGroupPlain = 0x100u;
+// A plain group whose ops are all diag(u0, u1) * R with R real (an Rz after an
+// Ry, for one): every op carries R in m[0] = (r00, r01), m[1] = (r10, r11),
+// and the head op carries in m[4 + i] the product of the ops' row phases for
+// element i of the group (bit k of i = the value of op k's target).
+constexpr uint32_t kGroupRot = 0x200u;
 constexpr int kMaxTileQubits = 13;  // 2^13 * 16 B = 128 KiB of LDS
 
 struct PassDesc {
@@ -543,6 +548,55 @@ __device__ __forceinline__ void plain_group(double2* tile, const TileMap<KQ>& P,
     }
 }
 
+// Real rotation R of a kGroupRot op along item axis AX: one multiply and one
+// FMA per output component.
+template <int AX>
+__device__ __forceinline__ void rot_axis(double2 (&a)[8], const DevOp& o) {
+    const double r00 = o.m[0].x, r01 = o.m[0].y, r10 = o.m[1].x, r11 = o.m[1].y;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i & (1 << AX)) continue;
+        const int j = i | (1 << AX);
+        const double2 x = a[i], y = a[j];
+        a[i] = make_double2(fma(r01, y.x, r00 * x.x), fma(r01, y.y, r00 * x.y));
+        a[j] = make_double2(fma(r11, y.x, r10 * x.x), fma(r11, y.y, r10 * x.y));
+    }
+}
+
+// A kGroupRot group: the ops' real rotations, then every element times the
+// product of the ops' row phases (the phases of op k act on axis k only, so
+// they commute past the rotations of the other axes).  16 FP64 instructions
+// per element for three ops against 24 for three dense complex matrices.
+template <uint32_t T, int KQ>
+__device__ __forceinline__ void rot_group(double2* tile, const TileMap<KQ>& P, const DevOp* __restrict__ g,
+                                          uint32_t cnt) {
+    const uint32_t t0 = g[0].t, t1 = g[1].t;
+    uint32_t t2 = 0;
+    if (cnt > 2) {
+        t2 = g[2].t;
+    } else {
+        while (t2 == t0 || t2 == t1) ++t2;
+    }
+    const uint32_t lo = min(t0, min(t1, t2)), hi = max(t0, max(t1, t2));
+    const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
+    const uint32_t p0 = P.lin(1u << t0), p1 = P.lin(1u << t1), p2 = P.lin(1u << t2);
+    for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
+        const uint32_t pb = P(ins0_32(ins0_32(ins0_32(it, lo), mid), hi));
+        uint32_t pe[8];
+        double2 a[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            pe[i] = pb ^ ((i & 1) ? p0 : 0u) ^ ((i & 2) ? p1 : 0u) ^ ((i & 4) ? p2 : 0u);
+            a[i] = tile[pe[i]];
+        }
+        rot_axis<0>(a, g[0]);
+        rot_axis<1>(a, g[1]);
+        if (cnt > 2) rot_axis<2>(a, g[2]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) tile[pe[i]] = cmul(g[0].m[4 + i], a[i]);
+    }
+}
+
 template <int KQ>
 __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <= 11 ? 4 : 1))) void k_fused(double2* __restrict__ s, uint64_t bmask, uint32_t nlow,
                                                      uint32_t nt, const DevOp* __restrict__ ops, uint32_t nops) {
@@ -594,7 +648,9 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             // a group of one-qubit ops on distinct tile bits: one LDS round
             // trip for the whole group
             const uint32_t cnt = o.pad & kGroupLen;
-            if (o.pad & kGroupPlain)
+            if (o.pad & kGroupRot)
+                rot_group<T, KQ>(tile, P, ops + oi, cnt);
+            else if (o.pad & kGroupPlain)
                 plain_group<T, KQ>(tile, P, ops + oi, cnt);
             else
                 fused_group<T, KQ>(tile, P, base, ops + oi, cnt);
@@ -1425,6 +1481,26 @@ static void schedule_tile_ops(const std::vector<HostOp>& ops, std::vector<int>& 
     order.swap(out);
 }
 
+// m = diag(u0, u1) * R with R real, when each row of m is real up to a phase
+// (to 1e-14 relative): r = R row-major, u = the row phases.
+static bool split_row_phases(const double2* m, double r[4], double2 u[2]) {
+    for (int row = 0; row < 2; ++row) {
+        const double2 a = m[2 * row], b = m[2 * row + 1];
+        const double na = std::hypot(a.x, a.y), nb = std::hypot(b.x, b.y);
+        const double2 big = na >= nb ? a : b;
+        const double nbig = std::max(na, nb);
+        if (!(nbig > 0.0)) return false;
+        const double2 ph = make_double2(big.x / nbig, big.y / nbig);
+        const double ra = ph.x * a.x + ph.y * a.y, ia = ph.x * a.y - ph.y * a.x;  // conj(ph) * a
+        const double rb = ph.x * b.x + ph.y * b.y, ib = ph.x * b.y - ph.y * b.x;
+        if (std::fabs(ia) > 1e-14 * nbig || std::fabs(ib) > 1e-14 * nbig) return false;
+        r[2 * row] = ra;
+        r[2 * row + 1] = rb;
+        u[row] = ph;
+    }
+    return true;
+}
+
 // Mark the groups of a tile pass in its localized op array: the plain groups
 // the schedule formed (kGroupPlain), and runs of 2 or 3 other consecutive U1
 // ops with distinct tile targets (the head op's pad holds the run length);
@@ -1464,6 +1540,36 @@ static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count
         }
         dops[first + j].pad = len >= 2 ? (len | (allp ? kGroupPlain : 0u)) : 0u;
         j += len;
+    }
+    // plain groups whose ops all split into row phases times a real rotation
+    for (uint32_t g = 0; g < count;) {
+        DevOp& head = dops[first + g];
+        const uint32_t len = head.pad & kGroupLen;
+        if (!(head.pad & kGroupPlain) || len < 2) {
+            ++g;
+            continue;
+        }
+        double r[3][4];
+        double2 u[3][2];
+        bool ok = true;
+        for (uint32_t k = 0; k < len && ok; ++k) ok = split_row_phases(dops[first + g + k].m, r[k], u[k]);
+        if (ok) {
+            for (uint32_t k = 0; k < len; ++k) {
+                DevOp& o = dops[first + g + k];
+                o.m[0] = make_double2(r[k][0], r[k][1]);
+                o.m[1] = make_double2(r[k][2], r[k][3]);
+            }
+            for (int i = 0; i < 8; ++i) {
+                double2 f = make_double2(1.0, 0.0);
+                for (uint32_t k = 0; k < len; ++k) {
+                    const double2 v = u[k][(i >> k) & 1];
+                    f = make_double2(f.x * v.x - f.y * v.y, f.x * v.y + f.y * v.x);
+                }
+                head.m[4 + i] = f;
+            }
+            head.pad |= kGroupRot;
+        }
+        g += len;
     }
 }
 
