@@ -548,13 +548,13 @@ __device__ __forceinline__ void plain_group(double2* tile, const TileMap<KQ>& P,
     }
 }
 
-// Real rotation R of a kGroupRot op along item axis AX: one multiply and one
-// FMA per output component.
-template <int AX>
-__device__ __forceinline__ void rot_axis(double2 (&a)[8], const DevOp& o) {
+// Real rotation R of a kGroupRot op along item axis AX of N elements: one
+// multiply and one FMA per output component.
+template <int AX, int N>
+__device__ __forceinline__ void rot_axis(double2 (&a)[N], const DevOp& o) {
     const double r00 = o.m[0].x, r01 = o.m[0].y, r10 = o.m[1].x, r11 = o.m[1].y;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < N; ++i) {
         if (i & (1 << AX)) continue;
         const int j = i | (1 << AX);
         const double2 x = a[i], y = a[j];
@@ -563,37 +563,77 @@ __device__ __forceinline__ void rot_axis(double2 (&a)[8], const DevOp& o) {
     }
 }
 
-// A kGroupRot group: the ops' real rotations, then every element times the
-// product of the ops' row phases (the phases of op k act on axis k only, so
-// they commute past the rotations of the other axes).  16 FP64 instructions
-// per element for three ops against 24 for three dense complex matrices.
-template <uint32_t T, int KQ>
+// A kGroupRot group of cnt <= G ops on G item axes (free tile bits pad the
+// axes of a shorter group): the ops' real rotations, then every element times
+// the product of the ops' row phases (the phases of op k act on axis k only,
+// so they commute past the rotations of the other axes).  For three ops, 16
+// FP64 instructions per element against 24 for three dense complex matrices.
+// Phase of element i: head m[4 + i] for i < 12, else m[i - 8] of the second op.
+template <uint32_t T, int KQ, int G>
 __device__ __forceinline__ void rot_group(double2* tile, const TileMap<KQ>& P, const DevOp* __restrict__ g,
                                           uint32_t cnt) {
-    const uint32_t t0 = g[0].t, t1 = g[1].t;
-    uint32_t t2 = 0;
-    if (cnt > 2) {
-        t2 = g[2].t;
-    } else {
-        while (t2 == t0 || t2 == t1) ++t2;
-    }
-    const uint32_t lo = min(t0, min(t1, t2)), hi = max(t0, max(t1, t2));
-    const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
-    const uint32_t p0 = P.lin(1u << t0), p1 = P.lin(1u << t1), p2 = P.lin(1u << t2);
-    for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
-        const uint32_t pb = P(ins0_32(ins0_32(ins0_32(it, lo), mid), hi));
-        uint32_t pe[8];
-        double2 a[8];
+    constexpr int N = 1 << G;
+    uint32_t t[G];
+    uint32_t used = 0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            pe[i] = pb ^ ((i & 1) ? p0 : 0u) ^ ((i & 2) ? p1 : 0u) ^ ((i & 4) ? p2 : 0u);
-            a[i] = tile[pe[i]];
+    for (int k = 0; k < G; ++k)
+        if ((uint32_t)k < cnt) used |= 1u << g[k].t;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if ((uint32_t)k < cnt) {
+            t[k] = g[k].t;
+        } else {
+            t[k] = (uint32_t)__builtin_ctz(~used);
+            used |= 1u << t[k];
         }
-        rot_axis<0>(a, g[0]);
-        rot_axis<1>(a, g[1]);
-        if (cnt > 2) rot_axis<2>(a, g[2]);
+    }
+    // insertion positions, ascending
+    uint32_t srt[G];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) tile[pe[i]] = cmul(g[0].m[4 + i], a[i]);
+    for (int k = 0; k < G; ++k) srt[k] = t[k];
+#pragma unroll
+    for (int x = 0; x < G; ++x)
+#pragma unroll
+        for (int y = 0; y + 1 < G - x; ++y) {
+            const uint32_t lo = min(srt[y], srt[y + 1]), hi = max(srt[y], srt[y + 1]);
+            srt[y] = lo;
+            srt[y + 1] = hi;
+        }
+    uint32_t pk[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) pk[k] = P.lin(1u << t[k]);
+    for (uint32_t it = threadIdx.x; it < T / N; it += kFusedTPB) {
+        uint32_t e = it;
+#pragma unroll
+        for (int k = 0; k < G; ++k) e = ins0_32(e, srt[k]);
+        const uint32_t pb = P(e);
+        double2 a[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            uint32_t q = pb;
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+                if (i & (1 << k)) q ^= pk[k];
+            a[i] = tile[q];
+        }
+        rot_axis<0, N>(a, g[0]);
+        rot_axis<1, N>(a, g[1]);
+        if (cnt > 2) rot_axis<2, N>(a, g[2]);
+        if constexpr (G > 3) {
+            if (cnt > 3) rot_axis<3, N>(a, g[3]);
+        }
+        // the store offsets are recomputed from an opaque copy of pb rather
+        // than kept live across the rotations (N registers fewer)
+        uint32_t pw = pb;
+        asm volatile("" : "+v"(pw));
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            uint32_t q = pw;
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+                if (i & (1 << k)) q ^= pk[k];
+            tile[q] = cmul(i < 12 ? g[0].m[4 + i] : g[1].m[i - 8], a[i]);
+        }
     }
 }
 
@@ -648,8 +688,10 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             // a group of one-qubit ops on distinct tile bits: one LDS round
             // trip for the whole group
             const uint32_t cnt = o.pad & kGroupLen;
-            if (o.pad & kGroupRot)
-                rot_group<T, KQ>(tile, P, ops + oi, cnt);
+            if ((o.pad & kGroupRot) && cnt > 3)
+                rot_group<T, KQ, 4>(tile, P, ops + oi, cnt);
+            else if (o.pad & kGroupRot)
+                rot_group<T, KQ, 3>(tile, P, ops + oi, cnt);
             else if (o.pad & kGroupPlain)
                 plain_group<T, KQ>(tile, P, ops + oi, cnt);
             else
@@ -1433,54 +1475,6 @@ static void merge_single_qubit_runs(std::vector<HostOp>& ops) {
 
 // A dense one-qubit gate with no condition (a plain group member).
 static inline bool plain_u1(const DevOp& o) { return o.kind == K_U1 && !o.hb && o.cmask == 0; }
-// An X with at most one condition bit: the tile's index map absorbs it
-// (cX, anti-controlled X), no amplitude moves.
-static inline bool map_op(const HostOp& h) {
-    return h.op.kind == K_U1 && h.op.hb && __builtin_popcountll(h.op.cmask) <= 1;
-}
-
-// Order the ops of a tile pass for the kernel: any order in which every op
-// follows each earlier op whose support it shares (ops with disjoint supports
-// commute), chosen greedily so that index-map ops come as early as they can
-// and dense unconditioned one-qubit gates on distinct qubits come in groups
-// of three (one LDS round trip each).  Returns the new order and the plain
-// groups as (position, length).
-static void schedule_tile_ops(const std::vector<HostOp>& ops, std::vector<int>& order,
-                              std::vector<std::pair<size_t, uint32_t>>& groups) {
-    std::vector<int> rem(order), out, ready, take, plain, other;
-    out.reserve(order.size());
-    groups.clear();
-    while (!rem.empty()) {
-        // ready: no earlier remaining op shares a qubit with it
-        ready.clear();
-        uint64_t seen = 0;
-        for (size_t i = 0; i < rem.size(); ++i) {
-            const uint64_t sup = ops[rem[i]].support;
-            if (!(sup & seen)) ready.push_back((int)i);
-            seen |= sup;
-        }
-        take.clear();
-        for (int i : ready)
-            if (map_op(ops[rem[i]])) take.push_back(i);
-        if (take.empty()) {
-            plain.clear();
-            other.clear();
-            for (int i : ready) (plain_u1(ops[rem[i]].op) ? plain : other).push_back(i);
-            if (plain.size() >= 3 || (other.empty() && plain.size() >= 2)) {
-                const size_t len = std::min<size_t>(3, plain.size());
-                groups.push_back({out.size(), (uint32_t)len});
-                take.assign(plain.begin(), plain.begin() + len);
-            } else {
-                take.push_back(other.empty() ? plain[0] : other[0]);
-            }
-        }
-        for (int i : take) out.push_back(rem[i]);
-        std::sort(take.begin(), take.end());
-        for (size_t k = take.size(); k-- > 0;) rem.erase(rem.begin() + take[k]);
-    }
-    order.swap(out);
-}
-
 // m = diag(u0, u1) * R with R real, when each row of m is real up to a phase
 // (to 1e-14 relative): r = R row-major, u = the row phases.
 static bool split_row_phases(const double2* m, double r[4], double2 u[2]) {
@@ -1499,6 +1493,67 @@ static bool split_row_phases(const double2* m, double r[4], double2 u[2]) {
         u[row] = ph;
     }
     return true;
+}
+
+// A plain U1 whose matrix splits into row phases times a real rotation.
+static inline bool rot_u1(const DevOp& o) {
+    double r[4];
+    double2 u[2];
+    return plain_u1(o) && split_row_phases(o.m, r, u);
+}
+// An X with at most one condition bit: the tile's index map absorbs it
+// (cX, anti-controlled X), no amplitude moves.
+static inline bool map_op(const HostOp& h) {
+    return h.op.kind == K_U1 && h.op.hb && __builtin_popcountll(h.op.cmask) <= 1;
+}
+
+// Order the ops of a tile pass for the kernel: any order in which every op
+// follows each earlier op whose support it shares (ops with disjoint supports
+// commute), chosen greedily so that index-map ops come as early as they can
+// and dense unconditioned one-qubit gates on distinct qubits come in groups
+// of three (one LDS round trip each).  Returns the new order and the plain
+// groups as (position, length).
+static void schedule_tile_ops(const std::vector<HostOp>& ops, std::vector<int>& order,
+                              std::vector<std::pair<size_t, uint32_t>>& groups) {
+    std::vector<int> rem(order), out, ready, take, plain, other, rot;
+    out.reserve(order.size());
+    groups.clear();
+    while (!rem.empty()) {
+        // ready: no earlier remaining op shares a qubit with it
+        ready.clear();
+        uint64_t seen = 0;
+        for (size_t i = 0; i < rem.size(); ++i) {
+            const uint64_t sup = ops[rem[i]].support;
+            if (!(sup & seen)) ready.push_back((int)i);
+            seen |= sup;
+        }
+        take.clear();
+        for (int i : ready)
+            if (map_op(ops[rem[i]])) take.push_back(i);
+        if (take.empty()) {
+            plain.clear();
+            other.clear();
+            for (int i : ready) (plain_u1(ops[rem[i]].op) ? plain : other).push_back(i);
+            rot.clear();
+            for (int i : plain)
+                if (rot_u1(ops[rem[i]].op)) rot.push_back(i);
+            if (rot.size() >= 4) {
+                // four rotations share one round trip of 16 elements per thread
+                groups.push_back({out.size(), 4u});
+                take.assign(rot.begin(), rot.begin() + 4);
+            } else if (plain.size() >= 3 || (other.empty() && plain.size() >= 2)) {
+                const size_t len = std::min<size_t>(3, plain.size());
+                groups.push_back({out.size(), (uint32_t)len});
+                take.assign(plain.begin(), plain.begin() + len);
+            } else {
+                take.push_back(other.empty() ? plain[0] : other[0]);
+            }
+        }
+        for (int i : take) out.push_back(rem[i]);
+        std::sort(take.begin(), take.end());
+        for (size_t k = take.size(); k-- > 0;) rem.erase(rem.begin() + take[k]);
+    }
+    order.swap(out);
 }
 
 // Mark the groups of a tile pass in its localized op array: the plain groups
@@ -1549,23 +1604,33 @@ static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count
             ++g;
             continue;
         }
-        double r[3][4];
-        double2 u[3][2];
+        double r[4][4];
+        double2 u[4][2];
         bool ok = true;
         for (uint32_t k = 0; k < len && ok; ++k) ok = split_row_phases(dops[first + g + k].m, r[k], u[k]);
+        if (!ok && len > 3) {
+            // the dense group path takes at most three ops: the fourth runs alone
+            head.pad = 3u | kGroupPlain;
+            dops[first + g + 3].pad = 0;
+            g += 3;
+            continue;
+        }
         if (ok) {
             for (uint32_t k = 0; k < len; ++k) {
                 DevOp& o = dops[first + g + k];
                 o.m[0] = make_double2(r[k][0], r[k][1]);
                 o.m[1] = make_double2(r[k][2], r[k][3]);
             }
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < (len > 3 ? 16 : 8); ++i) {
                 double2 f = make_double2(1.0, 0.0);
                 for (uint32_t k = 0; k < len; ++k) {
                     const double2 v = u[k][(i >> k) & 1];
                     f = make_double2(f.x * v.x - f.y * v.y, f.x * v.y + f.y * v.x);
                 }
-                head.m[4 + i] = f;
+                if (i < 12)
+                    head.m[4 + i] = f;
+                else
+                    dops[first + g + 1].m[i - 8] = f;
             }
             head.pad |= kGroupRot;
         }
