@@ -404,26 +404,35 @@ __device__ __forceinline__ void group_axis(double2 (&a)[8], const uint32_t (&e)[
 // of a GF(2) product over every tile bit.
 template <int KQ>
 struct TileMap {
-    uint32_t col[KQ];
+    // column b of M in lane b of one vector register (every wave keeps the
+    // same copy): a column at a runtime bit is one v_readlane and an update
+    // a lane compare and select, where columns in scalar registers need a
+    // compare chain over every tile bit
+    uint32_t colv;
+    uint32_t lane;
     uint32_t off;
     bool perm;   // false: P is the identity (no translation)
     bool dirty;  // the lookup tables are older than the map
     const uint32_t* tlo;  // LDS: P on index bits 0-5, with off
     const uint32_t* thi;  // LDS: M on index bits 6 and up
     __device__ __forceinline__ void init(const uint32_t* lo, const uint32_t* hi) {
-#pragma unroll
-        for (int b = 0; b < KQ; ++b) col[b] = 1u << b;
+        lane = threadIdx.x & (kWave - 1);
+        colv = lane < (uint32_t)KQ ? 1u << lane : 0u;
         off = 0;
         perm = false;
         dirty = false;
         tlo = lo;
         thi = hi;
     }
-    // M v from the register columns, bit by bit (table builds)
+    // column b (b uniform)
+    __device__ __forceinline__ uint32_t column(uint32_t b) const {
+        return (uint32_t)__builtin_amdgcn_readlane((int)colv, (int)b);
+    }
+    // M v from the columns, bit by bit (table builds)
     __device__ __forceinline__ uint32_t lin_cols(uint32_t v) const {
         uint32_t r = 0;
 #pragma unroll
-        for (int b = 0; b < KQ; ++b) r ^= ((v >> b) & 1u) ? col[b] : 0u;
+        for (int b = 0; b < KQ; ++b) r ^= ((v >> b) & 1u) ? column((uint32_t)b) : 0u;
         return r;
     }
     // M v = P(v) ^ P(0) from the lookup tables (after refresh): two uniform
@@ -431,21 +440,13 @@ struct TileMap {
     __device__ __forceinline__ uint32_t lin(uint32_t v) const { return perm ? (tlo[v & 63u] ^ thi[v >> 6] ^ tlo[0]) : v; }
     // P(l) = M l ^ off: linear over GF(2), so it splits into two lookups
     __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return perm ? (tlo[l & 63u] ^ thi[l >> 6]) : l; }
-    // column of tile bit t (t uniform): a compare chain keeps col[] in registers
-    __device__ __forceinline__ uint32_t column(uint32_t t) const {
-        uint32_t r = 0;
-#pragma unroll
-        for (int b = 0; b < KQ; ++b) r = ((uint32_t)b == t) ? col[b] : r;
-        return r;
-    }
     // G = X on t, controlled by tile bit c when cmask != 0 (cval: required value)
     __device__ __forceinline__ void apply_swap(uint32_t t, uint32_t cmask, uint32_t cval) {
         const uint32_t ct = column(t);
         if (cmask) {
             const uint32_t c = (uint32_t)__builtin_ctz(cmask);
-#pragma unroll
-            for (int b = 0; b < KQ; ++b)
-                if ((uint32_t)b == c) col[b] ^= ct;
+            const uint32_t nc = column(c) ^ ct;
+            colv = lane == c ? nc : colv;
             if (!(cval & cmask)) off ^= ct;
         } else {
             off ^= ct;
