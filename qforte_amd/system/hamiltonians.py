@@ -53,15 +53,17 @@ def build_sq_hamiltonian(h, g, constant=0.0):
                 ka, kb = 2 * k, 2 * k + 1
                 for l in range(norb):
                     la, lb = 2 * l, 2 * l + 1
+                    # one term per spin block, mixed blocks first, each
+                    # skipped when a creator or annihilator pair repeats
                     v = g[i, l, k, j] / 2
                     if ia != jb and kb != la:
-                        H.add(v, [ia, jb], [kb, la])  # abba
+                        H.add(v, [ia, jb], [kb, la])
                     if ib != ja and ka != lb:
-                        H.add(v, [ib, ja], [ka, lb])  # baab
+                        H.add(v, [ib, ja], [ka, lb])
                     if ia != ja and ka != la:
-                        H.add(v, [ia, ja], [ka, la])  # aaaa
+                        H.add(v, [ia, ja], [ka, la])
                     if ib != jb and kb != lb:
-                        H.add(v, [ib, jb], [kb, lb])  # bbbb
+                        H.add(v, [ib, jb], [kb, lb])
     return H
 
 

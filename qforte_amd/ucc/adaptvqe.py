@@ -86,38 +86,35 @@ class ADAPTVQE(UCCVQE):
 
     def update_ansatz(self):
         """Append the operator with the largest gradient (adaptvqe.py:380-458)."""
-        self._n_pauli_measures_k = 0
         grads = self.measure_gradient3()
-        curr_norm = 0.0
-        lgrst_grad, lgrst_idx = 0.0, None
-        secnd_grad, secnd_idx = 0.0, None
-        for m, g in enumerate(grads):
-            self._n_pauli_measures_k += self._Nl * self._Nm[m]
-            curr_norm += g * g
-            if abs(g) > abs(lgrst_grad):
-                if abs(lgrst_grad) > 0.0:
-                    secnd_grad, secnd_idx = lgrst_grad, lgrst_idx
-                lgrst_grad, lgrst_idx = g, m
-        self._curr_grad_norm = float(np.sqrt(curr_norm))
+        self._n_pauli_measures_k = self._Nl * sum(self._Nm[m] for m in range(len(grads)))
+        self._curr_grad_norm = float(np.sqrt(sum(g * g for g in grads)))
         self._grad_norms.append(self._curr_grad_norm)
+        # operators whose |gradient| set a new strict maximum in pool order:
+        # the last one is the choice; the one before it, if any, is the
+        # candidate that add_equiv_ops admits when their magnitudes tie
+        records, top = [], 0.0
+        for m, g in enumerate(grads):
+            if abs(g) > top:
+                top = abs(g)
+                records.append(m)
         self.conv_status()
         if self._converged:
             return
         if self._use_cumulative_thresh:
-            temp = []
-            gsq = sorted((grads[m] * grads[m], m) for m in range(len(grads)))
-            acc = 0.0
-            for gm_sq, m in gsq:
-                acc += gm_sq
-                if acc > self._avqe_thresh * self._avqe_thresh:
-                    self._tamps.append(0.0)
-                    temp.insert(0, m)
-            self._tops.extend(temp)
+            # the operators left once the smallest squared gradients summing to
+            # at most thresh^2 are dropped, largest first
+            order = sorted(range(len(grads)), key=lambda m: (grads[m] * grads[m], m))
+            running = np.cumsum([grads[m] * grads[m] for m in order])
+            chosen = [m for m, r in zip(order, running) if r > self._avqe_thresh * self._avqe_thresh][::-1]
+            self._tops.extend(chosen)
+            self._tamps.extend([0.0] * len(chosen))
         else:
-            self._tops.append(lgrst_idx)
+            best = records[-1] if records else None
+            self._tops.append(best)
             self._tamps.append(0.0)
-            if self._add_equiv_ops and secnd_idx is not None and abs(lgrst_grad) - abs(secnd_grad) < 1.0e-5:
-                self._tops.append(secnd_idx)
+            if self._add_equiv_ops and len(records) > 1 and abs(grads[best]) - abs(grads[records[-2]]) < 1.0e-5:
+                self._tops.append(records[-2])
                 self._tamps.append(0.0)
         self._n_classical_params_lst.append(len(self._tops))
 
