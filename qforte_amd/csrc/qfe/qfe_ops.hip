@@ -1512,8 +1512,9 @@ static inline bool map_op(const HostOp& h) {
 // follows each earlier op whose support it shares (ops with disjoint supports
 // commute), chosen greedily so that index-map ops come as early as they can
 // and dense unconditioned one-qubit gates on distinct qubits come in groups
-// of three (one LDS round trip each).  Returns the new order and the plain
-// groups as (position, length).
+// (one LDS round trip each): four when four ready gates each split into row
+// phases times a real rotation, else up to three.  Returns the new order and
+// the plain groups as (position, length).
 static void schedule_tile_ops(const std::vector<HostOp>& ops, std::vector<int>& order,
                               std::vector<std::pair<size_t, uint32_t>>& groups) {
     std::vector<int> rem(order), out, ready, take, plain, other, rot;
@@ -1558,10 +1559,12 @@ static void schedule_tile_ops(const std::vector<HostOp>& ops, std::vector<int>& 
 }
 
 // Mark the groups of a tile pass in its localized op array: the plain groups
-// the schedule formed (kGroupPlain), and runs of 2 or 3 other consecutive U1
-// ops with distinct tile targets (the head op's pad holds the run length);
-// the fused kernel applies each group with one LDS round trip.  Index-map
-// ops stay alone (absorbed without moving amplitudes).
+// the schedule formed (kGroupPlain; kGroupRot as well when every op splits
+// into row phases times a real rotation, the only form a group of four
+// takes), and runs of 2 or 3 other consecutive U1 ops with distinct tile
+// targets (the head op's pad holds the run length); the fused kernel applies
+// each group with one LDS round trip.  Index-map ops stay alone (absorbed
+// without moving amplitudes).
 static void group_u1_runs(std::vector<DevOp>& dops, size_t first, uint32_t count, int k,
                           const std::vector<std::pair<size_t, uint32_t>>& plain) {
     for (uint32_t j = 0; j < count; ++j) dops[first + j].pad = 0;
