@@ -518,6 +518,134 @@ def cpu_baseline(W, n_cpu, oracle_ref_loader, qf=None):
     }
 
 
+def _median_ms(fn, reps, warm=2):
+    import numpy as np
+
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        a = time.perf_counter()
+        fn()
+        ts.append(1e3 * (time.perf_counter() - a))
+    return float(np.median(ts))
+
+
+def small_configs_leg(qf, W, oracle_ref_loader):
+    """The two small configurations, end to end through the Python API (host
+    synchronisation included), next to the reference C++ Computer
+    (oracle/_ref, one thread) on the same inputs:
+    C1: 12 qubits from |0..0>, random_circuit(12, 500, seed=7), then <H> of
+        random_pauli_sum(12, 200, seed=8) (the inputs of
+        tests/golden/c1_reference.npz);
+    C2: the 16-qubit UCCSD energy (8 orbitals, 4a+4b, synthetic integrals seed
+        2024, SD pool: 360 amplitudes, 2688 Pauli exponentials, amplitudes
+        N(0, 0.05) seed 16) through the full-space program, its captured graph
+        and the sector template; the reference applies the same exponentials
+        as its own gate ladders and takes <H> of the JW Hamiltonian."""
+    import importlib.util
+
+    import numpy as np
+
+    ref = oracle_ref_loader()
+    spec = importlib.util.spec_from_file_location("sv_oracle", os.path.join(ROOT, "oracle", "sv_oracle.py"))
+    oracle = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(oracle)
+
+    def terms_of(op):
+        return [(c, [(g.gate_id(), g.target()) for g in circ.gates()]) for c, circ in op.terms()]
+
+    def ladders(mod, exps):
+        c = mod.Circuit()
+        for coef, term in exps:
+            for g in oracle.exponentiate_gates(coef, term):
+                c.add(mod.gate(g.gtype, g.target, g.control, g.parameter) if g.gtype in ("Rx", "Ry", "Rz")
+                      else mod.gate(g.gtype, g.target, g.control))
+        return c
+
+    out = {}
+    # C1
+    n = 12
+    gates = W.random_circuit(n, 500, seed=7)
+    terms = W.random_pauli_sum(n, 200, seed=8)
+    circ, H = W.to_circuit(qf, gates), W.to_operator(qf, terms)
+    qc = qf.Computer(n)
+
+    def c1_gpu():
+        qc.reset()
+        qc.apply_circuit(circ)
+        return qc.direct_op_exp_val(H)
+
+    c1 = {"gpu_ms": round(_median_ms(c1_gpu, 20), 3)}
+    e_gpu = c1_gpu()
+    amps = np.asarray(qc.get_coeff_vec())
+    if ref is not None:
+        rc, rH = W.to_circuit(ref, gates), W.to_operator(ref, terms)
+        rq = ref.Computer(n)
+        a = time.perf_counter()
+        rq.apply_circuit(rc)
+        e_ref = rq.direct_op_exp_val(rH)
+        c1["reference_cpu_ms"] = round(1e3 * (time.perf_counter() - a), 3)
+        c1["speedup"] = round(c1["reference_cpu_ms"] / c1["gpu_ms"], 1)
+        ref_amps = np.array(rq.get_coeff_vec(), dtype=np.complex128)
+        c1["energy_abs_diff"] = float(abs(complex(e_gpu) - complex(e_ref)))
+        c1["amplitude_max_abs_diff"] = float(np.max(np.abs(amps - ref_amps)))
+    out["c1_12q_500_gates_200_terms"] = c1
+    del qc
+    # C2
+    norb, nq = 8, 16
+    h, g = qf.synthetic_integrals(norb, 2024)
+    Hq = qf.build_sq_hamiltonian(h, g, 0.0).jw_transform()
+    occ = qf.system.hamiltonians.hf_occupation(norb, 4, 4)
+    pool = qf.SQOpPool()
+    pool.set_orb_spaces(occ)
+    pool.fill_pool("SD")
+    tops = list(range(len(pool)))
+    gens, _, _ = qf.ucc_generators(pool, tops)
+    t = list(np.random.default_rng(16).normal(0.0, 0.05, len(tops)))
+    exps = [(t[k] * c, term) for k, g_ in enumerate(gens) for c, term in terms_of(g_)]
+    a = time.perf_counter()
+    prog = qf.compile_ucc_program(nq, occ, Hq, pool, tops)
+    build_prog_ms = 1e3 * (time.perf_counter() - a)
+    hmap = qf.HamiltonianMap(norb)
+    inputs = hmap.pack(0.0, h, g)
+    a = time.perf_counter()
+    tpl = qf.compile_ucc_template(nq, occ, hmap, pool, tops)
+    build_tpl_ms = 1e3 * (time.perf_counter() - a)
+    c2 = {"exponentials": len(exps), "hamiltonian_terms": len(Hq.terms()),
+          "gpu_full_space_ms": round(_median_ms(lambda: prog.energy(t), 20), 3),
+          "gpu_full_space_graph_ms": round(_median_ms(lambda: prog.energy_graph(t), 20), 3),
+          "gpu_sector_ms": round(_median_ms(lambda: tpl.energy(t, inputs), 20), 3),
+          "build_ms": {"full_space": round(build_prog_ms, 1), "sector": round(build_tpl_ms, 1)}}
+    e_prog, e_graph, e_tpl = prog.energy(t), prog.energy_graph(t), tpl.energy(t, inputs)
+    c2["energy"] = e_prog
+    c2["graph_minus_eager"] = float(abs(e_graph - e_prog))
+    c2["sector_minus_full_space"] = float(abs(e_tpl - e_prog))
+    if ref is not None:
+        rc = ref.Circuit()
+        for q in range(nq):
+            if occ[q]:
+                rc.add(ref.gate("X", q, q))
+        rc.add(ladders(ref, exps))
+        rH = ref.QubitOperator()
+        for c, term in terms_of(Hq):
+            cc = ref.Circuit()
+            for letter, q in term:
+                cc.add(ref.gate(letter, q, q))
+            rH.add(c, cc)
+        rq = ref.Computer(nq)
+        a = time.perf_counter()
+        rq.apply_circuit(rc)
+        e_ref = rq.direct_op_exp_val(rH)
+        c2["reference_cpu_ms"] = round(1e3 * (time.perf_counter() - a), 1)
+        c2["reference_gates"] = rc.size()
+        c2["speedup_full_space"] = round(c2["reference_cpu_ms"] / c2["gpu_full_space_ms"], 1)
+        c2["speedup_sector"] = round(c2["reference_cpu_ms"] / c2["gpu_sector_ms"], 1)
+        c2["energy_abs_diff_vs_reference"] = float(abs(complex(e_ref).real - e_prog))
+    out["c2_16q_uccsd_energy"] = c2
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -533,6 +661,7 @@ def main():
     ap.add_argument("--tuning", type=str, default="", help="tile_qubits,low_qubits,max_ops (skips the autotuner)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default fusion width and tile shape")
     ap.add_argument("--no-serving", action="store_true", help="skip the UCC serving leg")
+    ap.add_argument("--no-small", action="store_true", help="skip the 12- and 16-qubit configurations")
     ap.add_argument("--no-gradient", action="store_true", help="skip the 20-qubit gradient leg")
     ap.add_argument("--gradient-only", action="store_true", help="only the 20-qubit gradient leg (printed as JSON)")
     ap.add_argument("--serving-only", action="store_true", help="only the UCC serving leg (printed as JSON)")
@@ -687,6 +816,14 @@ def main():
     gradient = None
     if not args.no_gradient:
         gradient = gradient_leg(qf, W)
+    small = None
+    if not args.no_small:
+        print("[bench] 12- and 16-qubit configurations", file=sys.stderr, flush=True)
+        loader = lambda: None  # noqa: E731
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from conftest import load_reference as loader
+        small = small_configs_leg(qf, W, loader)
     line = {
         "metric": METRIC,
         "value": round(gbps, 1),
@@ -722,6 +859,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "gradient": gradient,
+        "small_configs": small,
     }
     print(json.dumps(line))
 

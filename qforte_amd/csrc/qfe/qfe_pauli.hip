@@ -211,6 +211,12 @@ int launch_apply_sum_async(const double2* in, double2* out, int n, const PauliGr
 // pass of the state then serves up to 16 groups.
 // ---------------------------------------------------------------------------
 constexpr int kEvOrbitMax = 4;
+// Smallest state that takes orbit passes: below it one launch over every
+// group (a read of the state per group) is faster than the chain of orbit
+// passes, whose workgroups then cover too little of the state (measured on
+// MI355X, 200 and 1000 random terms: 0.25 vs 0.50 and 0.96 vs 1.91 ms at 20
+// qubits, 1.25 vs 0.89 and 5.2 vs 3.5 ms at 22).
+constexpr int kEvOrbitMinQubits = 21;
 constexpr int kEvOrbitGroups = 16;
 
 struct OrbitGroup {
@@ -506,7 +512,7 @@ int expval_groups(const double2* bra, const double2* ket, int n, const std::vect
     Context& cx = ctx();
     const size_t G = groups.size();
     if (G == 0) return QFE_OK;
-    if (n >= kEvOrbitMax + 8) return expval_groups_orbit(bra, ket, n, groups, zs, cs, out_per_group);
+    if (n >= kEvOrbitMinQubits) return expval_groups_orbit(bra, ket, n, groups, zs, cs, out_per_group);
     const uint32_t nchunk = chunks_for(n, G);
     const size_t b_part = align256(G * nchunk * sizeof(double2));
     const size_t b_out = align256(G * sizeof(double2));
@@ -584,7 +590,7 @@ static int expval_groups_fixed(const double2* bra, const double2* ket, int n, co
     if (G == 0) return QFE_OK;
     FixedLayout A{static_cast<char*>(dev_scratch), kNoAllocBytes};
     std::vector<double2> res(G);
-    if (n < kEvOrbitMax + 8) {
+    if (n < kEvOrbitMinQubits) {
         const uint32_t nchunk = chunks_for(n, G);
         char* dg = A.take(G * sizeof(PauliGroup));
         char* dz = A.take(zs.size() * sizeof(uint64_t));

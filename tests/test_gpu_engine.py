@@ -346,6 +346,26 @@ def test_pool_expectations_with_multipliers(qf, oracle):
         qc.direct_oppl_exp_val_w_mults(pool, mults[:2])
 
 
+def test_orbit_expectation_passes_at_21_qubits(qf, oracle):
+    # from 21 qubits <psi|H|psi> packs the X-mask groups into orbit passes
+    # (below, one launch reads the state once per group); complex
+    # coefficients, an identity term, masks on the lowest qubits and a
+    # repeated mask
+    n = 21
+    terms = W.random_pauli_sum(n, 60, seed=44, wmax=6, real_coeffs=False)
+    terms += [(0.4, [("X", 0)]), (0.2j, [("Y", 0), ("X", 1)]), (-0.3, [("X", 1), ("Y", 2), ("Z", 5)]), (0.7, [])]
+    terms += [(0.25, terms[0][1])]
+    psi = rand_state(n, 9)
+    qc = qf.Computer(n)
+    qc.set_coeff_vec(psi)
+    qf.reset_counters()
+    e = qc.direct_op_exp_val(W.to_operator(qf, terms))
+    assert qf.get_counters()["reduce_passes"] < 50  # orbit passes, not one per group
+    # the oracle's mask form (pinned against the gate-by-gate form in tests/test_oracle.py)
+    expect = oracle.expval_masks(psi, [(c,) + tuple(oracle.pauli_masks(t)) for c, t in terms])
+    assert abs(e - expect) < E_TOL
+
+
 @pytest.mark.parametrize("n", [4, 11, 17])
 def test_operator_application_and_expectations(qf, oracle, n):
     terms = W.random_pauli_sum(n, 60, seed=41, wmax=min(6, n), real_coeffs=False)
