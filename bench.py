@@ -542,11 +542,22 @@ def small_configs_leg(qf, W, oracle_ref_loader):
         2024, SD pool: 360 amplitudes, 2688 Pauli exponentials, amplitudes
         N(0, 0.05) seed 16) through the full-space program, its captured graph
         and the sector template; the reference applies the same exponentials
-        as its own gate ladders and takes <H> of the JW Hamiltonian."""
+        as its own gate ladders and takes <H> of the JW Hamiltonian.
+    Both run with the library's default tile shape (chosen by state size),
+    not the shape tuned for the 30-qubit workload."""
     import importlib.util
 
     import numpy as np
 
+    saved_tuning = qf.get_tuning()
+    qf.set_tuning(-1, -1, -1)
+    try:
+        return _small_configs(qf, W, oracle_ref_loader, importlib, np)
+    finally:
+        qf.set_tuning(*saved_tuning)
+
+
+def _small_configs(qf, W, oracle_ref_loader, importlib, np):
     ref = oracle_ref_loader()
     spec = importlib.util.spec_from_file_location("sv_oracle", os.path.join(ROOT, "oracle", "sv_oracle.py"))
     oracle = importlib.util.module_from_spec(spec)

@@ -82,8 +82,11 @@ int qfe_get_stream(void** stream);
 /* Block until all queued work finished. */
 int qfe_synchronize(void);
 /* Fusion / launch tuning: max tile qubits (k), low contiguous qubits (L) and
- * max ops per fused pass.  Zero leaves a value unchanged.  No reference
- * counterpart (autotuner hook). */
+ * max ops per fused pass.  Zero leaves a value unchanged; -1 (any argument)
+ * restores the default, a shape chosen by state size ((8, 4, 64) up to 14
+ * qubits, (10, 4, 32) at 15-20, (11, 4, 24) above; register-orbit passes
+ * take at most 24 ops), for which qfe_get_tuning reports -1 -1 -1.  No
+ * reference counterpart (autotuner hook). */
 int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass);
 int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass);
 /* Largest register orbit of the planner, 1..7 dimensions (5, 6 and 7 spread one

@@ -24,12 +24,32 @@ int hip_fail(hipError_t e, const char* what);
 
 // ---- library context -----------------------------------------------------------
 struct Tuning {
+    bool tile_auto = true; // tile shape chosen by state size (tile_shape); false: the three values below
     int tile_qubits = 11;  // k: qubits held by one fused tile (2^k amplitudes in LDS)
     int low_qubits = 4;    // L: lowest qubits always in the tile (contiguous 2^L runs)
     int max_ops = 24;      // ops per fused pass before LDS traffic dominates
     int orbit_dims = 7;    // max dimensions of a register orbit (5, 6, 7: spread over 2, 4 or 8 lanes)
     uint64_t sector_words = 1ull << 25;  // max pair words of one sector template (256 MB)
 };
+
+// Pass shape for a state of n qubits: the explicit tuning, or by default the
+// shape measured fastest for the size on MI355X.  A small state needs small
+// tiles so that a pass has enough workgroups and each op a short per-thread
+// chain: (8, 4, 64) up to 14 qubits (500 random gates at 12 qubits: 0.59 vs
+// 0.93 ms), (10, 4, 32) at 15-20 (the 16-qubit UCCSD energy, 2688
+// exponentials: 2.73 ms, against 3.38 ms with 8-qubit tiles), (11, 4, 24)
+// from 21 on.  Register-orbit passes keep 24 ops: longer ones are bound by
+// their arithmetic while the state sits in cache (the same UCCSD energy with
+// 64: 4.85 ms).
+struct TileShape {
+    int k, low, max_ops, orbit_ops;
+};
+inline TileShape tile_shape(const Tuning& tu, int n) {
+    if (!tu.tile_auto) return {tu.tile_qubits, tu.low_qubits, tu.max_ops, tu.max_ops};
+    if (n <= 14) return {8, 4, 64, 24};
+    if (n <= 20) return {10, 4, 32, 24};
+    return {11, 4, 24, 24};
+}
 
 struct Counters {
     uint64_t fused = 0, single = 0, reduce = 0;

@@ -1204,8 +1204,9 @@ static void grow_orbit(const std::vector<HostOp>& ops, const std::vector<char>& 
 
 static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::vector<Pass>& passes,
                  bool device_angles) {
-    const int K = std::min(tu.tile_qubits, n);
-    const int L = std::max(std::min(tu.low_qubits, K), K - 9);
+    const TileShape shape = tile_shape(tu, n);
+    const int K = std::min(shape.k, n);
+    const int L = std::max(std::min(shape.low, K), K - 9);
     const uint64_t all = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
     const uint64_t lowmask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
     std::vector<char> done(ops.size(), 0);
@@ -1214,10 +1215,13 @@ static void plan(const std::vector<HostOp>& ops, int n, const Tuning& tu, std::v
         while (first < ops.size() && done[first]) ++first;
         if (first >= ops.size()) break;
         Pass tile, orbit;
-        grow_tile(ops, done, first, K, lowmask, all, tu.max_ops, tile);
-        grow_orbit(ops, done, first, all, tu.max_ops, orbit, P_ORBIT, std::max(1, std::min(tu.orbit_dims, kLaneOrbitMax)));
+        grow_tile(ops, done, first, K, lowmask, all, shape.max_ops, tile);
+        grow_orbit(ops, done, first, all, shape.orbit_ops, orbit, P_ORBIT, std::max(1, std::min(tu.orbit_dims, kLaneOrbitMax)));
         Pass p;
-        if (orbit.ops.size() >= 2 && orbit.ops.size() >= tile.ops.size() && n > (int)orbit.span.b.size()) {
+        // an orbit pass at its op limit wins over a longer tile pass (tiles may
+        // take more ops than orbits when the shape is chosen by state size)
+        const size_t rival = std::min(tile.ops.size(), (size_t)shape.orbit_ops);
+        if (orbit.ops.size() >= 2 && orbit.ops.size() >= rival && n > (int)orbit.span.b.size()) {
             p = std::move(orbit);
         } else if (tile.ops.size() >= 2) {
             p = std::move(tile);
@@ -1716,7 +1720,7 @@ int plan_build(std::vector<HostOp>& ops_in, int n, bool device_angles, OpPlan** 
             continue;
         }
         // extend the tile with the lowest free qubits up to K bits
-        const int K = std::min(cx.tuning.tile_qubits, n);
+        const int K = std::min(tile_shape(cx.tuning, n).k, n);
         uint64_t b = p.bmask;
         for (int q = 0; q < n && __builtin_popcountll(b) < K; ++q) b |= 1ull << q;
         PassDesc pd{};

@@ -365,6 +365,18 @@ int qfe_synchronize(void) {
 
 int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass) {
     Tuning t = g_ctx.tuning;
+    if (tile_qubits == -1 || low_qubits == -1 || max_ops_per_pass == -1) {
+        // the default: the tile shape follows the state size
+        const Tuning d;
+        t.tile_auto = true;
+        t.tile_qubits = d.tile_qubits;
+        t.low_qubits = d.low_qubits;
+        t.max_ops = d.max_ops;
+        g_ctx.tuning = t;
+        return QFE_OK;
+    }
+    if (!tile_qubits && !low_qubits && !max_ops_per_pass) return QFE_OK;
+    t.tile_auto = false;
     if (tile_qubits) t.tile_qubits = tile_qubits;
     if (low_qubits) t.low_qubits = low_qubits;
     if (max_ops_per_pass) t.max_ops = max_ops_per_pass;
@@ -376,9 +388,10 @@ int qfe_set_tuning(int tile_qubits, int low_qubits, int max_ops_per_pass) {
 }
 
 int qfe_get_tuning(int* tile_qubits, int* low_qubits, int* max_ops_per_pass) {
-    if (tile_qubits) *tile_qubits = g_ctx.tuning.tile_qubits;
-    if (low_qubits) *low_qubits = g_ctx.tuning.low_qubits;
-    if (max_ops_per_pass) *max_ops_per_pass = g_ctx.tuning.max_ops;
+    const bool a = g_ctx.tuning.tile_auto;
+    if (tile_qubits) *tile_qubits = a ? -1 : g_ctx.tuning.tile_qubits;
+    if (low_qubits) *low_qubits = a ? -1 : g_ctx.tuning.low_qubits;
+    if (max_ops_per_pass) *max_ops_per_pass = a ? -1 : g_ctx.tuning.max_ops;
     return QFE_OK;
 }
 

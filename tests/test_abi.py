@@ -47,6 +47,25 @@ def test_error_reporting_without_device_work():
         assert b"no GPU" in lib.qfe_last_error()
 
 
+def test_tile_tuning_default_follows_the_state_size():
+    lib = ctypes.CDLL(LIB)
+    get = lambda: [v.value for v in vals] if lib.qfe_get_tuning(*[ctypes.byref(v) for v in vals]) == 0 else None  # noqa: E731
+    vals = [ctypes.c_int(0) for _ in range(3)]
+    saved = get()
+    try:
+        assert lib.qfe_set_tuning(-1, -1, -1) == 0
+        assert get() == [-1, -1, -1]  # the default: shape chosen by state size
+        assert lib.qfe_set_tuning(0, 0, 0) == 0 and get() == [-1, -1, -1]
+        assert lib.qfe_set_tuning(10, 0, 0) == 0
+        assert get() == [10, 4, 24]  # explicit; zeros keep the stored values
+        assert lib.qfe_set_tuning(0, 0, 32) == 0 and get() == [10, 4, 32]
+        assert lib.qfe_set_tuning(20, 0, 0) == 1  # rejected, state unchanged
+        assert get() == [10, 4, 32]
+        assert lib.qfe_set_tuning(-1, 0, 0) == 0 and get() == [-1, -1, -1]
+    finally:
+        lib.qfe_set_tuning(*saved)
+
+
 def test_python_computer_fails_loudly_without_gpu(qf):
     if qf.device_count() > 0:
         pytest.skip("a GPU is present")
