@@ -107,7 +107,8 @@ struct DevOp {
     uint32_t kind;
     uint32_t t;       // target bit (global) or local bit inside a fused tile
     uint32_t c;       // second bit for K_U2
-    uint32_t hb;      // K_PEXP: highest set bit of x (global or local)
+    uint32_t hb;      // K_PEXP: highest set bit of x (global or local); K_U1 / K_U2: 1 when the matrix
+                      // is an X / a qubit swap (a permutation a tile's index map absorbs)
     uint64_t x;       // K_PEXP: flip mask
     uint64_t z;       // K_PEXP: sign mask (in-tile part when fused)
     uint64_t cmask;   // condition mask (in-tile part when fused)

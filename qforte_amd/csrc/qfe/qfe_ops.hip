@@ -44,6 +44,18 @@ static inline uint32_t is_swap(const double2* m) {
                : 0u;
 }
 
+// The two-qubit swap in the |c t> basis (rows and columns 2c + t): exactly
+// 1 at (0,0), (1,2), (2,1), (3,3), 0 elsewhere.
+static inline uint32_t is_qubit_swap(const double2* m) {
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            const bool one = (r == c && (r == 0 || r == 3)) || (r == 1 && c == 2) || (r == 2 && c == 1);
+            const double2 v = m[4 * r + c];
+            if (v.y != 0.0 || v.x != (one ? 1.0 : 0.0)) return 0u;
+        }
+    return 1u;
+}
+
 static void add_u1(int t, const qfe_c64 u[4], uint64_t cmask, std::vector<HostOp>& out) {
     // u = {m00, m01, m10, m11}; classified like computer.cc:353-422
     const bool diag_nz = cabs(u[0]) + cabs(u[3]) > kComputeThreshold;
@@ -93,6 +105,7 @@ int lower_gate(const qfe_gate& g, int n, std::vector<HostOp>& out) {
     h.op.t = (uint32_t)t;
     h.op.c = (uint32_t)c;
     for (int i = 0; i < 16; ++i) h.op.m[i] = d2(g.m[i]);
+    h.op.hb = is_qubit_swap(h.op.m);  // a permutation a tile's index map absorbs
     h.mix = (1ull << t) | (1ull << c);
     h.support = h.mix;
     out.push_back(h);
@@ -440,6 +453,13 @@ struct TileMap {
     __device__ __forceinline__ uint32_t lin(uint32_t v) const { return perm ? (tlo[v & 63u] ^ thi[v >> 6] ^ tlo[0]) : v; }
     // P(l) = M l ^ off: linear over GF(2), so it splits into two lookups
     __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return perm ? (tlo[l & 63u] ^ thi[l >> 6]) : l; }
+    // G = SWAP of tile bits a and b: M' e_a = M e_b and M' e_b = M e_a
+    __device__ __forceinline__ void swap_cols(uint32_t a, uint32_t b) {
+        const uint32_t ca = column(a), cb = column(b);
+        colv = lane == a ? cb : (lane == b ? ca : colv);
+        perm = true;
+        dirty = true;
+    }
     // G = X on t, controlled by tile bit c when cmask != 0 (cval: required value)
     __device__ __forceinline__ void apply_swap(uint32_t t, uint32_t cmask, uint32_t cval) {
         const uint32_t ct = column(t);
@@ -681,6 +701,10 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             // a permutation of the tile: the map absorbs it (registers only,
             // so no barrier)
             if ((base & o.omask) == o.oval) P.apply_swap(o.t, (uint32_t)o.cmask, (uint32_t)o.cval);
+            continue;
+        }
+        if (o.kind == K_U2 && o.hb) {  // a qubit swap, absorbed the same way
+            P.swap_cols(o.t, o.c);
             continue;
         }
         __syncthreads();
@@ -1506,10 +1530,10 @@ static inline bool rot_u1(const DevOp& o) {
     double2 u[2];
     return plain_u1(o) && split_row_phases(o.m, r, u);
 }
-// An X with at most one condition bit: the tile's index map absorbs it
-// (cX, anti-controlled X), no amplitude moves.
+// An X with at most one condition bit, or a qubit swap: the tile's index map
+// absorbs it (cX, anti-controlled X, SWAP), no amplitude moves.
 static inline bool map_op(const HostOp& h) {
-    return h.op.kind == K_U1 && h.op.hb && __builtin_popcountll(h.op.cmask) <= 1;
+    return (h.op.kind == K_U1 && h.op.hb && __builtin_popcountll(h.op.cmask) <= 1) || (h.op.kind == K_U2 && h.op.hb);
 }
 
 // Order the ops of a tile pass for the kernel: any order in which every op
