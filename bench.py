@@ -271,6 +271,7 @@ def serving_leg(qf, W, nreq, plain_per_class):
         "speedup_vs_plain": round(capacity / plain_rate, 1) if plain_rate else None,
         "speedup_vs_plain_sector": round(capacity / sector_rate, 1) if sector_rate else None,
         "plain_ms_per_eval_by_n": {n: round(1e3 * v, 3) for n, v in plain_full.items()},
+        "plain_sector_ms_per_eval_by_n": {n: round(1e3 * v, 3) for n, v in plain_sector.items()},
         "plain_graph_replay_ms_by_n": {n: round(1e3 * v, 3) for n, v in plain_graph.items()},
         "plain_graph_capture_ms_by_n": {n: round(1e3 * v, 2) for n, v in capture.items()},
         "plain_program_build_ms_by_n": {n: round(1e3 * v, 2) for n, v in create_full.items()},

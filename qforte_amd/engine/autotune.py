@@ -2,7 +2,7 @@
 
 The planner has three knobs (``set_tuning``: LDS tile qubits k, contiguous low
 qubits L, ops per fused pass; ``set_pass_tuning``: the widest register orbit,
-4, 5 or 6 dimensions).  Which setting is fastest depends on the op mix and on
+4 to 7 dimensions).  Which setting is fastest depends on the op mix and on
 the state size, so the tuner times candidates on the caller's own circuits,
 applied to a replica state of ``probe_qubits`` qubits (the circuits must act
 only on those qubits), with HIP events on the engine stream, and keeps the
