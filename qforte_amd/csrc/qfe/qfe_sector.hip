@@ -1886,18 +1886,25 @@ struct qfe_batch_s {
 namespace {
 
 // Requests of different tiers never share a launch, so small requests do not
-// wait for the longest request of a mixed launch.  One lane per tier, on three
-// streams the library creates once and every engine shares: with the library
-// stream these are HIP's default four hardware queues per process, so a lane
-// never shares a queue (and waits behind another lane's kernel).  Two lanes
-// per tier (six streams on four queues) measured 13.4 ms p50 at half load
-// against 1.65 ms with one, at the same capacity.
+// wait for the longest request of a mixed launch.  One lane per tier: tier 0
+// on the library stream, tiers 1 and 2 on two streams the library creates once
+// and every engine shares.  With the null stream (synchronous copies, torch)
+// these are four streams, HIP's default four hardware queues per process, so
+// no lane shares a queue with another.  Streams that share a queue run one
+// kernel at a time between them: with a third lane stream of its own (five
+// streams on four queues) the 16-qubit requests of tier 1 waited behind the
+// tier-2 launches (p50 13.6 ms at half load, 1.2 ms without), and two lanes
+// per tier (six lane streams) gave 13.4 ms p50 over all classes.
 constexpr int kLanesPerTier = 1;
 constexpr int kLanes = 3 * kLanesPerTier;
 
 hipStream_t g_lane_stream[kLanes] = {};
 
 int lane_stream(int i, hipStream_t* out) {
+    if (i == 0) {  // the tier-0 lane runs on the library stream
+        *out = ctx().stream;
+        return QFE_OK;
+    }
     if (!g_lane_stream[i]) QFE_HIP(hipStreamCreateWithFlags(&g_lane_stream[i], hipStreamNonBlocking));
     *out = g_lane_stream[i];
     return QFE_OK;
