@@ -197,7 +197,7 @@ struct EvPass {
     std::vector<uint32_t> groups;  // indices into the group list
     std::vector<uint64_t> basis;   // reduced basis, pivot = highest bit
 };
-void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes);
+void plan_expval(const std::vector<PauliGroup>& groups, int maxdim, std::vector<EvPass>& passes);
 
 // device address of the fixed expectation scratch on `device` (current)
 int noalloc_scratch_setup(int device);

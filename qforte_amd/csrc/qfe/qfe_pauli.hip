@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -205,12 +206,16 @@ int launch_apply_sum_async(const double2* in, double2* out, int n, const PauliGr
 
 // ---------------------------------------------------------------------------
 // Orbit-fused expectation passes.  Groups whose X masks span <= 4 dimensions
-// over GF(2) are evaluated together: every thread loads the 2^M amplitudes
+// over GF(2) (5 for a plain expectation value) are evaluated together: every thread loads the 2^M amplitudes
 // of its orbit {j ^ c_i} once and accumulates, for each group g with mask
 // v_g in orbit coordinates, sum_i conj(bra_i) ket_{i^v} f_g(i^v).  One read
 // pass of the state then serves up to 16 groups.
 // ---------------------------------------------------------------------------
 constexpr int kEvOrbitMax = 4;
+// Same-state passes (bra == ket) go one dimension wider: the pair symmetry of
+// their products leaves them memory-bound at 5 dimensions.
+constexpr int kEvOrbitMaxSame = 5;
+static_assert((1 << kEvOrbitMaxSame) <= 32, "OrbitTerm sign masks hold 32 elements");
 // Smallest state that takes orbit passes: below it one launch over every
 // group (a read of the state per group) is faster than the chain of orbit
 // passes, whose workgroups then cover too little of the state (measured on
@@ -256,9 +261,10 @@ __device__ __forceinline__ void partner_products(uint32_t v, double2 (&p)[1 << M
     }
 }
 
-// SAME: bra and ket are one state (plain expectation value): load it once.
-template <int M, bool SAME, bool NT>
-__global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(SAME ? 3 : 2))) void k_orbit_expval(const double2* __restrict__ bra,
+// Cross expectation pass <bra|H|ket> (bra != ket; a plain expectation value
+// takes k_orbit_expval_same).
+template <int M, bool NT>
+__global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(2))) void k_orbit_expval(const double2* __restrict__ bra,
                                                          const double2* __restrict__ ket, uint64_t norbit,
                                                          uint64_t b0, uint64_t b1, uint64_t b2, uint64_t b3,
                                                          uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3,
@@ -288,11 +294,7 @@ __global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(SAME ? 3
             for (int k = 0; k < M; ++k)
                 if ((i >> k) & 1) c ^= bb[k];
             a[i] = ld_s<NT>(ket + (j ^ c));
-            if constexpr (!SAME) b[i] = ld_s<NT>(bra + (j ^ c));
-        }
-        if constexpr (SAME) {
-#pragma unroll
-            for (int i = 0; i < (1 << M); ++i) b[i] = a[i];
+            b[i] = ld_s<NT>(bra + (j ^ c));
         }
         for (uint32_t g = 0; g < ngroups; ++g) {
             const OrbitGroup G = groups[g];
@@ -326,67 +328,324 @@ __global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(SAME ? 3
     }
 }
 
-// Pack groups into passes whose X masks span <= 4 dimensions (first fit).
-void plan_expval(const std::vector<PauliGroup>& groups, std::vector<EvPass>& passes) {
-    std::vector<char> used(groups.size(), 0);
-    for (size_t s = 0; s < groups.size(); ++s) {
-        if (used[s]) continue;
-        EvPass p;
-        for (size_t g = s; g < groups.size() && p.groups.size() < (size_t)kEvOrbitGroups; ++g) {
-            if (used[g]) continue;
-            // reduce x against the basis; accept if it stays within kEvOrbitMax
-            uint64_t r = groups[g].x;
-            for (uint64_t v : p.basis)
-                if ((r >> (63 - __builtin_clzll(v))) & 1ull) r ^= v;
-            if (r) {
-                if ((int)p.basis.size() >= kEvOrbitMax) continue;
-                const int piv = 63 - __builtin_clzll(r);
-                for (uint64_t& v : p.basis)
-                    if ((v >> piv) & 1ull) v ^= r;
-                p.basis.push_back(r);
-            }
-            used[g] = 1;
-            p.groups.push_back((uint32_t)g);
+// Basis of an orbit pass: vectors and their pivots, ascending.
+struct OrbitBasis {
+    uint64_t b[kEvOrbitMaxSame];
+    uint32_t p[kEvOrbitMaxSame];
+};
+
+// Sign vector of a term in orbit coordinates: bit k = bit 2^k of smask (the
+// sign of element i is parity(i & zm), linear in i).
+template <int M>
+__device__ __forceinline__ uint32_t term_sign_vector(uint32_t smask) {
+    uint32_t zm = 0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) zm |= ((smask >> (1u << k)) & 1u) << k;
+    return zm;
+}
+
+// Same-state expectation pass (bra == ket).  For a group with flip v the
+// products p_i = conj(a_i) a_{i ^ v} pair up as p_{i ^ v} = conj(p_i), and a
+// term's signs s_i (bit i of pmask) satisfy s_{i ^ v} = sigma s_i with sigma
+// = (-1)^(bit v of smask), because the sign pattern over an orbit is linear
+// in the element index.  So, with h the highest bit of v,
+//   sum_i s_i p_i = 2 sum_{i: bit h clear} s_i Re p_i    (sigma = +1)
+//                 = 2i sum_{i: bit h clear} s_i Im p_i   (sigma = -1):
+// two products and an FMA per pair of elements instead of a complex product
+// and two FMAs per element.  A diagonal group (v = 0, the terms with no X or
+// Y; one per operator; diag is the first in the pass, ngroups if none) sums
+// s_i |a_i|^2 = H[zm], the Walsh-Hadamard transform of q_i = |a_i|^2
+// at the term's sign vector zm (bit k of zm: bit 2^k of smask, since the
+// sign pattern is linear in i): it runs after the other groups, transforms
+// q in place of the amplitudes once, and then costs each of its terms (often
+// the most numerous group) one register select instead of a sum over the
+// orbit.  Further groups with v = 0 in the same pass (one per operator of
+// qfe_expval_many) sum s_i |a_i|^2 directly.
+template <int M, bool NT>
+__global__ __launch_bounds__(kEvTPB) __attribute__((amdgpu_waves_per_eu(3))) void k_orbit_expval_same(
+    const double2* __restrict__ psi, uint64_t norbit, OrbitBasis ob, const OrbitGroup* __restrict__ groups,
+    uint32_t ngroups, uint32_t diag, const OrbitTerm* __restrict__ terms, double2* __restrict__ partial, uint32_t nb) {
+    extern __shared__ double2 acc_s[];
+    __shared__ double2 lds[16];
+    for (uint32_t g = 0; g < ngroups; ++g) acc_s[g * kEvTPB + threadIdx.x] = make_double2(0.0, 0.0);
+    const uint64_t ch = xcd_block();
+    const uint64_t per = ((norbit + nb - 1) / nb + kEvTPB - 1) / kEvTPB * kEvTPB;
+    const uint64_t t1 = min(norbit, (ch + 1) * per);
+    for (uint64_t t = ch * per + threadIdx.x; t < t1; t += kEvTPB) {
+        uint64_t j = t;
+#pragma unroll
+        for (int k = 0; k < M; ++k) j = ins0(j, ob.p[k]);
+        double2 a[1 << M];
+#pragma unroll
+        for (int i = 0; i < (1 << M); ++i) {
+            uint64_t c = 0;
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+                if ((i >> k) & 1) c ^= ob.b[k];
+            a[i] = ld_s<NT>(psi + (j ^ c));
         }
+        for (uint32_t g = 0; g < ngroups; ++g) {
+            if (g == diag) continue;
+            const OrbitGroup G = groups[g];
+            double2 acc = make_double2(0.0, 0.0);
+            dispatch_mask<M>(G.v, [&](auto vc) {
+                constexpr uint32_t V = decltype(vc)::value;
+                for (uint32_t tt = 0; tt < G.count; ++tt) {
+                    const OrbitTerm T = terms[G.first + tt];
+                    double w = 0.0;
+                    bool imag = false;
+                    // the empty volatile asm on each partner keeps its products
+                    // inside this branch and this term: otherwise the compiler
+                    // computes the products of every branch up front, out of the
+                    // term loop, and spills them
+                    if constexpr (V == 0) {
+                        // a second diagonal group in one pass (one per operator
+                        // of qfe_expval_many): s_i |a_i|^2 summed directly
+#pragma unroll
+                        for (int i = 0; i < (1 << M); ++i) {
+                            double2 q = a[i];
+                            asm volatile("" : "+v"(q.x), "+v"(q.y));
+                            const double sg = ((T.pmask >> i) & 1u) ? -1.0 : 1.0;
+                            w = fma(sg, fma(q.x, q.x, q.y * q.y), w);
+                        }
+                    } else {
+                        constexpr uint32_t H = 1u << (31 - __builtin_clz(V));
+                        imag = (T.smask >> V) & 1u;
+                        if (!imag) {
+#pragma unroll
+                            for (int i = 0; i < (1 << M); ++i) {
+                                if (i & H) continue;
+                                double2 q = a[i ^ V];
+                                asm volatile("" : "+v"(q.x), "+v"(q.y));
+                                const double sg = ((T.pmask >> i) & 1u) ? -1.0 : 1.0;
+                                w = fma(sg, fma(a[i].x, q.x, a[i].y * q.y), w);
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < (1 << M); ++i) {
+                                if (i & H) continue;
+                                double2 q = a[i ^ V];
+                                asm volatile("" : "+v"(q.x), "+v"(q.y));
+                                const double sg = ((T.pmask >> i) & 1u) ? -1.0 : 1.0;
+                                w = fma(sg, fma(a[i].x, q.y, -(a[i].y * q.x)), w);
+                            }
+                        }
+                        w *= 2.0;
+                    }
+                    if (__popcll(j & T.z) & 1) w = -w;
+                    // c * w (sigma = +1) or c * i w (sigma = -1)
+                    if (imag)
+                        acc = make_double2(fma(-T.c.y, w, acc.x), fma(T.c.x, w, acc.y));
+                    else
+                        acc = make_double2(fma(T.c.x, w, acc.x), fma(T.c.y, w, acc.y));
+                }
+            });
+            acc_s[g * kEvTPB + threadIdx.x] = cadd(acc_s[g * kEvTPB + threadIdx.x], acc);
+        }
+        if (diag < ngroups) {
+            // q_i = |a_i|^2 over the real parts, then H[k] = sum_i (-1)^popcount(i & k) q_i in place
+#pragma unroll
+            for (int i = 0; i < (1 << M); ++i) a[i].x = fma(a[i].x, a[i].x, a[i].y * a[i].y);
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+#pragma unroll
+                for (int i = 0; i < (1 << M); ++i) {
+                    if (i & (1 << k)) continue;
+                    const double u = a[i].x, v = a[i | (1 << k)].x;
+                    a[i].x = u + v;
+                    a[i | (1 << k)].x = u - v;
+                }
+            }
+            const OrbitGroup G = groups[diag];
+            double2 acc = make_double2(0.0, 0.0);
+            for (uint32_t tt = 0; tt < G.count; ++tt) {
+                const OrbitTerm T = terms[G.first + tt];
+                const uint32_t zm = term_sign_vector<M>(T.smask);
+                double w = 0.0;
+                // one branch per value of zm (an empty volatile asm keeps the
+                // branches apart, so the select is never a variable index into
+                // the register array, which would move it to scratch memory)
+                dispatch_mask<M>(zm, [&](auto zc) {
+                    double t = a[decltype(zc)::value].x;
+                    asm volatile("" : "+v"(t));
+                    w = t;
+                });
+                if (__popcll(j & T.z) & 1) w = -w;
+                acc = make_double2(fma(T.c.x, w, acc.x), fma(T.c.y, w, acc.y));
+            }
+            acc_s[diag * kEvTPB + threadIdx.x] = cadd(acc_s[diag * kEvTPB + threadIdx.x], acc);
+        }
+    }
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        const double2 s = block_sum(acc_s[g * kEvTPB + threadIdx.x], lds);
+        if (threadIdx.x == 0) partial[(uint64_t)groups[g].slot * nb + ch] = s;
+    }
+}
+
+// Pack groups into passes whose X masks span <= maxdim dimensions.  A
+// pass grows its basis one vector at a time: each remaining mask is kept
+// reduced by the basis so far (its coset representative, 0 once the mask
+// lies in the span), and the next vector is the representative shared by the
+// most remaining groups (fewest bits, then earliest group, on ties), which
+// brings all of them into the span at once.  A representative whose highest
+// bit, the new pivot, is below kNtMinBit is taken only when no other is left:
+// a pass with such a pivot has every lane of a load touch every other
+// amplitude and runs 1.7x longer (4.8 vs 2.8 ms at 30 qubits), so the masks
+// that need one end up together.  The groups whose masks then lie in the span
+// join the pass, in group order, up to kEvOrbitGroups.  On 1000 random terms
+// of weight 1-6 over 30 qubits (647 X masks) this packs 92 passes of 5
+// dimensions, one of them with a low pivot (91 passes, 13 with low pivots,
+// without the pivot rule; 126 taking the groups in order, first fit).
+void plan_expval(const std::vector<PauliGroup>& groups, int maxdim, std::vector<EvPass>& passes) {
+    std::vector<uint32_t> rem(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) rem[g] = (uint32_t)g;
+    std::vector<uint64_t> red(groups.size());
+    // open-addressing tally of representatives: key, count, first position,
+    // and the step that wrote the slot (slots of older steps read as empty)
+    size_t cap = 16;
+    while (cap < 2 * groups.size()) cap <<= 1;
+    std::vector<uint64_t> key(cap);
+    std::vector<uint32_t> count(cap), first(cap), stamp(cap, 0);
+    uint32_t step = 0;
+    while (!rem.empty()) {
+        EvPass p;
+        for (size_t i = 0; i < rem.size(); ++i) red[i] = groups[rem[i]].x;
+        for (;;) {
+            size_t inside = 0;
+            for (size_t i = 0; i < rem.size(); ++i) inside += red[i] == 0;
+            if (inside >= (size_t)kEvOrbitGroups || (int)p.basis.size() >= maxdim) break;
+            ++step;
+            uint64_t best = 0;
+            uint32_t bc = 0, bf = 0;
+            for (size_t i = 0; i < rem.size(); ++i) {
+                const uint64_t r = red[i];
+                if (!r) continue;
+                size_t h = (size_t)((r * 0x9E3779B97F4A7C15ull) >> 32) & (cap - 1);
+                while (stamp[h] == step && key[h] != r) h = (h + 1) & (cap - 1);
+                if (stamp[h] != step) {
+                    stamp[h] = step;
+                    key[h] = r;
+                    count[h] = 0;
+                    first[h] = (uint32_t)i;
+                }
+                // every representative passes through each count up to its
+                // final one, so the running best ends as the maximum
+                const uint32_t c = ++count[h];
+                const int pc = __builtin_popcountll(r), bpc = __builtin_popcountll(best);
+                const bool hi = (r >> kNtMinBit) != 0, bhi = (best >> kNtMinBit) != 0;
+                if (!best || (hi && !bhi) ||
+                    (hi == bhi && (c > bc || (c == bc && (pc < bpc || (pc == bpc && first[h] < bf)))))) {
+                    best = r;
+                    bc = c;
+                    bf = first[h];
+                }
+            }
+            if (!best) break;
+            // best has no bit in a pivot column of the basis; its own pivot
+            // is cleared from the other vectors and from every representative
+            const int piv = 63 - __builtin_clzll(best);
+            for (uint64_t& v : p.basis)
+                if ((v >> piv) & 1ull) v ^= best;
+            p.basis.push_back(best);
+            for (size_t i = 0; i < rem.size(); ++i)
+                if ((red[i] >> piv) & 1ull) red[i] ^= best;
+        }
+        size_t w = 0;
+        for (size_t i = 0; i < rem.size(); ++i) {
+            if (red[i] == 0 && p.groups.size() < (size_t)kEvOrbitGroups)
+                p.groups.push_back(rem[i]);
+            else
+                rem[w++] = rem[i];
+        }
+        rem.resize(w);
         passes.push_back(std::move(p));
     }
 }
 
+// Plans of the last few operators, keyed by their X masks in group order:
+// callers that evaluate one operator many times (a variational loop, a step
+// of a benchmark) plan it once.  Planning 647 masks takes about 2 ms on the
+// host, against 5-10 us per saved pass at 21 qubits.
+namespace {
+struct PlanEntry {
+    int maxdim = 0;
+    std::vector<uint64_t> masks;
+    std::vector<EvPass> passes;
+    uint64_t used = 0;
+};
+std::mutex g_plan_mu;
+std::vector<PlanEntry> g_plans;
+uint64_t g_plan_clock = 0;
+constexpr size_t kPlanCacheEntries = 8;
+}  // namespace
+
+static void cached_plan(const std::vector<PauliGroup>& groups, int maxdim, std::vector<EvPass>& passes) {
+    std::vector<uint64_t> masks(groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) masks[g] = groups[g].x;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        for (PlanEntry& e : g_plans)
+            if (e.maxdim == maxdim && e.masks == masks) {
+                e.used = ++g_plan_clock;
+                passes = e.passes;
+                return;
+            }
+    }
+    passes.clear();
+    plan_expval(groups, maxdim, passes);
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    if (g_plans.size() < kPlanCacheEntries) {
+        g_plans.push_back(PlanEntry{maxdim, std::move(masks), passes, ++g_plan_clock});
+        return;
+    }
+    PlanEntry* lru = &g_plans[0];
+    for (PlanEntry& e : g_plans)
+        if (e.used < lru->used) lru = &e;
+    *lru = PlanEntry{maxdim, std::move(masks), passes, ++g_plan_clock};
+}
+
 template <int M>
-static void launch_orbit_expval_k(const double2* bra, const double2* ket, uint64_t norbit, const uint64_t* b,
-                                  const uint32_t* piv, const OrbitGroup* dg, uint32_t ng, const OrbitTerm* dt,
+static void launch_orbit_expval_k(const double2* bra, const double2* ket, uint64_t norbit, const OrbitBasis& ob,
+                                  const OrbitGroup* dg, uint32_t ng, uint32_t diag, const OrbitTerm* dt,
                                   double2* partial, uint32_t nb, bool nt, hipStream_t st) {
     const size_t lds = (size_t)ng * kEvTPB * sizeof(double2);
-    if (bra == ket && nt)
-        hipLaunchKernelGGL((k_orbit_expval<M, true, true>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
-                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
-    else if (bra == ket)
-        hipLaunchKernelGGL((k_orbit_expval<M, true, false>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
-                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
-    else if (nt)
-        hipLaunchKernelGGL((k_orbit_expval<M, false, true>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
-                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
-    else
-        hipLaunchKernelGGL((k_orbit_expval<M, false, false>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0],
-                           b[1], b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    if (bra == ket) {
+        if (nt)
+            hipLaunchKernelGGL((k_orbit_expval_same<M, true>), dim3(nb), dim3(kEvTPB), lds, st, ket, norbit, ob, dg, ng,
+                               diag, dt, partial, nb);
+        else
+            hipLaunchKernelGGL((k_orbit_expval_same<M, false>), dim3(nb), dim3(kEvTPB), lds, st, ket, norbit, ob, dg, ng,
+                               diag, dt, partial, nb);
+        return;
+    }
+    if constexpr (M <= kEvOrbitMax) {
+        const uint64_t* b = ob.b;
+        const uint32_t* piv = ob.p;
+        if (nt)
+            hipLaunchKernelGGL((k_orbit_expval<M, true>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0], b[1],
+                               b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+        else
+            hipLaunchKernelGGL((k_orbit_expval<M, false>), dim3(nb), dim3(kEvTPB), lds, st, bra, ket, norbit, b[0], b[1],
+                               b[2], b[3], piv[0], piv[1], piv[2], piv[3], dg, ng, dt, partial, nb);
+    }
 }
 
 // One orbit expectation pass as planned on the host: its basis (pivots
 // ascending) and its run of OrbitGroup entries.
 struct OrbitLaunch {
     int M;
-    uint64_t b[4];
-    uint32_t piv[4];
+    OrbitBasis ob;
     size_t g0, ng;
+    uint32_t diag;  // position of the first group with v = 0 in the pass, ng if none
 };
 
 // Host tables of every pass: per pass its OrbitGroup run (slot = group index)
 // and OrbitTerm run, in pass order.
 static void build_orbit_tables(const std::vector<PauliGroup>& groups, const std::vector<uint64_t>& zs,
-                               const std::vector<double2>& cs, std::vector<OrbitGroup>& og, std::vector<OrbitTerm>& ot,
-                               std::vector<OrbitLaunch>& launches) {
+                               const std::vector<double2>& cs, int maxdim, std::vector<OrbitGroup>& og,
+                               std::vector<OrbitTerm>& ot, std::vector<OrbitLaunch>& launches) {
     std::vector<EvPass> passes;
-    plan_expval(groups, passes);
+    cached_plan(groups, maxdim, passes);
     og.clear();
     ot.clear();
     launches.clear();
@@ -396,10 +655,10 @@ static void build_orbit_tables(const std::vector<PauliGroup>& groups, const std:
         OrbitLaunch L{};
         L.M = (int)b.size();
         for (int k = 0; k < L.M; ++k) {
-            L.b[k] = b[k];
-            L.piv[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
+            L.ob.b[k] = b[k];
+            L.ob.p[k] = (uint32_t)(63 - __builtin_clzll(b[k]));
         }
-        uint64_t coff[1 << kEvOrbitMax];
+        uint64_t coff[1 << kEvOrbitMaxSame];
         for (int i = 0; i < (1 << L.M); ++i) {
             uint64_t c = 0;
             for (int k = 0; k < L.M; ++k)
@@ -408,14 +667,16 @@ static void build_orbit_tables(const std::vector<PauliGroup>& groups, const std:
         }
         L.g0 = og.size();
         L.ng = p.groups.size();
+        L.diag = (uint32_t)L.ng;
         for (uint32_t gi : p.groups) {
             const PauliGroup& pg = groups[gi];
             OrbitGroup o{};
             for (int k = 0; k < L.M; ++k)
-                if ((pg.x >> L.piv[k]) & 1ull) o.v |= 1u << k;
+                if ((pg.x >> L.ob.p[k]) & 1ull) o.v |= 1u << k;
             o.first = (uint32_t)ot.size();
             o.count = pg.count;
             o.slot = gi;
+            if (o.v == 0 && L.diag == L.ng) L.diag = (uint32_t)(og.size() - L.g0);
             og.push_back(o);
             for (uint32_t t = 0; t < pg.count; ++t) {
                 OrbitTerm T{};
@@ -447,13 +708,16 @@ static int launch_orbit_pass(const double2* bra, const double2* ket, int n, cons
     const uint64_t norbit = N >> L.M;
     const OrbitGroup* g = dg + L.g0;
     const uint32_t ng = (uint32_t)L.ng;
-    const bool nt = use_nt(n, L.M == 0 ? 63u : L.piv[0]);  // pivots ascending
+    const bool nt = use_nt(n, L.M == 0 ? 63u : L.ob.p[0]);  // pivots ascending
+    if (L.M > (bra == ket ? kEvOrbitMaxSame : kEvOrbitMax))
+        return fail(QFE_ERR_STATE, "internal: expectation orbit dimension out of range");
     switch (L.M) {
-        case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-        case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-        case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-        case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
-        case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.b, L.piv, g, ng, dt, partial, nb, nt, cx.stream); break;
+        case 0: launch_orbit_expval_k<0>(bra, ket, norbit, L.ob, g, ng, L.diag, dt, partial, nb, nt, cx.stream); break;
+        case 1: launch_orbit_expval_k<1>(bra, ket, norbit, L.ob, g, ng, L.diag, dt, partial, nb, nt, cx.stream); break;
+        case 2: launch_orbit_expval_k<2>(bra, ket, norbit, L.ob, g, ng, L.diag, dt, partial, nb, nt, cx.stream); break;
+        case 3: launch_orbit_expval_k<3>(bra, ket, norbit, L.ob, g, ng, L.diag, dt, partial, nb, nt, cx.stream); break;
+        case 4: launch_orbit_expval_k<4>(bra, ket, norbit, L.ob, g, ng, L.diag, dt, partial, nb, nt, cx.stream); break;
+        case 5: launch_orbit_expval_k<5>(bra, ket, norbit, L.ob, g, ng, L.diag, dt, partial, nb, nt, cx.stream); break;
         default: return fail(QFE_ERR_STATE, "internal: expectation orbit dimension out of range");
     }
     QFE_HIP(hipGetLastError());
@@ -472,7 +736,7 @@ static int expval_groups_orbit(const double2* bra, const double2* ket, int n, co
     std::vector<OrbitGroup> og;
     std::vector<OrbitTerm> ot;
     std::vector<OrbitLaunch> launches;
-    build_orbit_tables(groups, zs, cs, og, ot, launches);
+    build_orbit_tables(groups, zs, cs, bra == ket ? kEvOrbitMaxSame : kEvOrbitMax, og, ot, launches);
     const size_t b_g = align256(og.size() * sizeof(OrbitGroup));
     const size_t b_t = align256(ot.size() * sizeof(OrbitTerm));
     const size_t b_part = align256(G * (size_t)nb * sizeof(double2));
@@ -618,7 +882,7 @@ static int expval_groups_fixed(const double2* bra, const double2* ket, int n, co
     std::vector<OrbitGroup> og;
     std::vector<OrbitTerm> ot;
     std::vector<OrbitLaunch> launches;
-    build_orbit_tables(groups, zs, cs, og, ot, launches);
+    build_orbit_tables(groups, zs, cs, bra == ket ? kEvOrbitMaxSame : kEvOrbitMax, og, ot, launches);
     char* dg = A.take(og.size() * sizeof(OrbitGroup));
     char* dt = A.take(ot.size() * sizeof(OrbitTerm));
     char* dout = A.take(G * sizeof(double2));

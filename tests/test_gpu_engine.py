@@ -348,13 +348,23 @@ def test_pool_expectations_with_multipliers(qf, oracle):
 
 def test_orbit_expectation_passes_at_21_qubits(qf, oracle):
     # from 21 qubits <psi|H|psi> packs the X-mask groups into orbit passes
-    # (below, one launch reads the state once per group); complex
-    # coefficients, an identity term, masks on the lowest qubits and a
-    # repeated mask
+    # of up to 5 dimensions (below, one launch reads the state once per
+    # group); complex coefficients, an identity term, masks on the lowest
+    # qubits, a repeated mask, a diagonal group of many Z strings (taken by
+    # its Walsh-Hadamard transform) and groups of several X/Y variants of one
+    # mask (both the real and the imaginary pair sums)
     n = 21
+    rng = np.random.default_rng(45)
     terms = W.random_pauli_sum(n, 60, seed=44, wmax=6, real_coeffs=False)
     terms += [(0.4, [("X", 0)]), (0.2j, [("Y", 0), ("X", 1)]), (-0.3, [("X", 1), ("Y", 2), ("Z", 5)]), (0.7, [])]
     terms += [(0.25, terms[0][1])]
+    for k in range(24):
+        qs = sorted(int(q) for q in rng.choice(n, size=1 + k % 5, replace=False))
+        terms.append((complex(rng.normal(), rng.normal()), [("Z", q) for q in qs]))
+    for letters in (("X", "X"), ("X", "Y"), ("Y", "X"), ("Y", "Y")):
+        for zq in (None, 2, 17):
+            t = [(letters[0], 4), (letters[1], 11)] + ([("Z", zq)] if zq is not None else [])
+            terms.append((complex(rng.normal(), rng.normal()), sorted(t, key=lambda lq: lq[1])))
     psi = rand_state(n, 9)
     qc = qf.Computer(n)
     qc.set_coeff_vec(psi)
@@ -362,8 +372,29 @@ def test_orbit_expectation_passes_at_21_qubits(qf, oracle):
     e = qc.direct_op_exp_val(W.to_operator(qf, terms))
     assert qf.get_counters()["reduce_passes"] < 50  # orbit passes, not one per group
     # the oracle's mask form (pinned against the gate-by-gate form in tests/test_oracle.py)
-    expect = oracle.expval_masks(psi, [(c,) + tuple(oracle.pauli_masks(t)) for c, t in terms])
+    masks = [(c,) + tuple(oracle.pauli_masks(t)) for c, t in terms]
+    expect = oracle.expval_masks(psi, masks)
     assert abs(e - expect) < E_TOL
+    # the same operator between two states (passes of up to 4 dimensions)
+    phi = rand_state(n, 10)
+    other = qf.Computer(n)
+    other.set_coeff_vec(phi)
+    got = qf.cross_expectations(other, qc, [W.to_operator(qf, terms)])[0]
+    want = sum(c * np.vdot(phi, oracle.apply_pauli(psi, x, z, ph)) for c, x, z, ph in masks)
+    assert abs(got - want) < E_TOL
+    # several operators in one call, each with its own diagonal group: the
+    # groups of all operators share passes, so one pass holds several
+    pool = qf.QubitOpPool()
+    subs = []
+    for k in range(3):
+        sub = [(0.5 + k, [("Z", k)]), (-0.25j, [("Z", 3), ("Z", 7 + k)]), (0.3, [("X", 2 + k), ("Y", 9)])]
+        sub += W.random_pauli_sum(n, 6, seed=200 + k, wmax=4)
+        subs.append(sub)
+        pool.add(1.0 + 0.5 * k, W.to_operator(qf, sub))
+    vals = qc.direct_oppl_exp_val(pool)
+    for k, sub in enumerate(subs):
+        ref = oracle.expval_masks(psi, [(c,) + tuple(oracle.pauli_masks(t)) for c, t in sub])
+        assert abs(vals[k] - (1.0 + 0.5 * k) * ref) < E_TOL
 
 
 @pytest.mark.parametrize("n", [4, 11, 17])
