@@ -222,6 +222,22 @@ def serving_leg(qf, W, nreq, plain_per_class):
         }
         print(f"[bench] serving at {f}x: {rates[f'{f}x']}", file=sys.stderr, flush=True)
 
+    # one request at a time through the serving engine: a class's service time
+    # with nothing else queued, the reference for its latency under load
+    alone, alone_diff = {}, 0.0
+    eng1 = qf.ServingEngine(hbm_budget_bytes=64 << 30, max_active=8192)
+    for n in sorted(classes):
+        lat1 = []
+        for i in [i for i, r in enumerate(reqs) if r[0] == n][:8]:
+            _, inputs, t = reqs[i]
+            rid = eng1.submit(classes[n]["template"], t, inputs)
+            eng1.drain()
+            lat1.append(eng1.latencies[rid])
+            alone_diff = max(alone_diff, abs(eng1.results[rid] - batched[i]))
+        alone[n] = float(np.median(lat1))
+    print(f"[bench] served alone, median ms by class: { {n: round(1e3 * v, 3) for n, v in alone.items()} }",
+          file=sys.stderr, flush=True)
+
     # plain paths on a sample: per class, the first plain_per_class requests
     by_class = {n: [i for i, r in enumerate(reqs) if r[0] == n][:plain_per_class] for n in classes}
     plain_full, plain_sector, create_full, plain_graph, capture, max_err = {}, {}, {}, {}, {}, 0.0
@@ -266,6 +282,8 @@ def serving_leg(qf, W, nreq, plain_per_class):
                      "synthetic integrals per request, amplitudes N(0,0.05)"),
         "capacity_evals_per_s": round(capacity, 1),
         "poisson": rates,
+        "served_alone_ms_by_n": {n: round(1e3 * v, 3) for n, v in alone.items()},
+        "served_alone_max_abs_diff_vs_capacity_run": alone_diff,
         "plain_full_space_evals_per_s": round(plain_rate, 2) if plain_rate else None,
         "plain_sector_one_launch_evals_per_s": round(sector_rate, 2) if sector_rate else None,
         "speedup_vs_plain": round(capacity / plain_rate, 1) if plain_rate else None,

@@ -397,6 +397,28 @@ def test_orbit_expectation_passes_at_21_qubits(qf, oracle):
         assert abs(vals[k] - (1.0 + 0.5 * k) * ref) < E_TOL
 
 
+def test_expectation_plans_are_cached_by_mask_not_by_coefficient(qf, oracle):
+    # plans are cached by the operator's X masks: the same masks with other
+    # coefficients must use the new coefficients, and more operators than the
+    # cache holds (eight) must each still evaluate correctly when repeated
+    n = 21
+    psi = rand_state(n, 11)
+    qc = qf.Computer(n)
+    qc.set_coeff_vec(psi)
+    base = W.random_pauli_sum(n, 40, seed=70, wmax=5, real_coeffs=False)
+    masks = [tuple(oracle.pauli_masks(t)) for _, t in base]
+    for scale in (1.0, -2.5j, 0.75):
+        terms = [(scale * c, t) for c, t in base]
+        e = qc.direct_op_exp_val(W.to_operator(qf, terms))
+        expect = oracle.expval_masks(psi, [(c,) + m for (c, _), m in zip(terms, masks)])
+        assert abs(e - expect) < E_TOL
+    ops = [W.random_pauli_sum(n, 12, seed=80 + k, wmax=4, real_coeffs=False) for k in range(10)]
+    want = [oracle.expval_masks(psi, [(c,) + tuple(oracle.pauli_masks(t)) for c, t in op]) for op in ops]
+    for _ in range(2):
+        for op, w in zip(ops, want):
+            assert abs(qc.direct_op_exp_val(W.to_operator(qf, op)) - w) < E_TOL
+
+
 @pytest.mark.parametrize("n", [4, 11, 17])
 def test_operator_application_and_expectations(qf, oracle, n):
     terms = W.random_pauli_sum(n, 60, seed=41, wmax=min(6, n), real_coeffs=False)
