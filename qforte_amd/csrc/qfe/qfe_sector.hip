@@ -1886,7 +1886,8 @@ struct qfe_batch_s {
 namespace {
 
 // Requests of different tiers never share a launch, so small requests do not
-// wait for the longest request of a mixed launch.  One lane per tier: tier 0
+// wait for the longest request of a mixed launch (nor, in tier 2, requests of
+// different sector dimensions).  One lane per tier: tier 0
 // on the library stream, tiers 1 and 2 on two streams the library creates once
 // and every engine shares.  With the null stream (synchronous copies, torch)
 // these are four streams, HIP's default four hardware queues per process, so
@@ -1894,7 +1895,11 @@ namespace {
 // kernel at a time between them: with a third lane stream of its own (five
 // streams on four queues) the 16-qubit requests of tier 1 waited behind the
 // tier-2 launches (p50 13.6 ms at half load, 1.2 ms without), and two lanes
-// per tier (six lane streams) gave 13.4 ms p50 over all classes.
+// per tier (six lane streams) gave 13.4 ms p50 over all classes.  Letting two
+// lanes take tier-1 and tier-2 launches by age instead (one dimension never on
+// both) brought the 18-qubit p50 from 7.7 to 3.6 ms but the 16-qubit one from
+// 1.2 to 2.5 ms, and two concurrent tier-2 launches slowed the tier-0 lane:
+// 21.2K instead of 21.7K energies/s, p50 45 instead of 3 ms at 1.5x load.
 constexpr int kLanesPerTier = 1;
 constexpr int kLanes = 3 * kLanesPerTier;
 
@@ -1945,18 +1950,33 @@ int lane_launch(qfe_batch b, Lane& L) {
     std::deque<Request>& qg = b->gqueue[L.tier];
     const bool grad = qe.empty() || (!qg.empty() && qg.front().id < qe.front().id);
     std::deque<Request>& queue = grad ? qg : qe;
-    // admit from the front of the queue while the footprint fits this lane's share
+    // admit from the front of the queue while the footprint fits this lane's
+    // share.  A tier-2 launch holds requests of one sector dimension, that of
+    // its oldest request: a tier-2 request runs its whole ansatz chain in one
+    // workgroup, and the launch ends with its longest chain (alone, 1.7 ms for
+    // an 18-qubit and 6.6 ms for a 20-qubit UCCSD energy), so shorter requests
+    // would otherwise wait for the longest; requests of other dimensions keep
+    // their place in the queue.
     const uint64_t share = std::min<uint64_t>(b->budget / kLanes, kLaunchFootprint);
+    const bool one_dim = L.tier == 2;
+    const uint64_t dim0 = queue.front().t->D;
     std::vector<Request> take;
+    std::vector<Request> skipped;
     uint64_t foot = 0;
     while (!queue.empty() && (int64_t)(b->active + take.size()) < b->max_active &&
            (!grad || take.size() < kMaxGradLaunch)) {
+        if (one_dim && queue.front().t->D != dim0) {
+            skipped.push_back(std::move(queue.front()));
+            queue.pop_front();
+            continue;
+        }
         const uint64_t f = grad ? queue.front().t->grad_request_bytes : queue.front().t->request_bytes;
         if (!take.empty() && foot + f > share) break;
         foot += f;
         take.push_back(std::move(queue.front()));
         queue.pop_front();
     }
+    for (size_t i = skipped.size(); i-- > 0;) queue.push_front(std::move(skipped[i]));
     if (take.empty()) return QFE_OK;
     // heaviest requests first, so their workgroups start first
     std::stable_sort(take.begin(), take.end(), [](const Request& x, const Request& y) { return x.t->cost > y.t->cost; });
