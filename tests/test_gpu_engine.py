@@ -412,6 +412,12 @@ def test_expectation_plans_are_cached_by_mask_not_by_coefficient(qf, oracle):
         e = qc.direct_op_exp_val(W.to_operator(qf, terms))
         expect = oracle.expval_masks(psi, [(c,) + m for (c, _), m in zip(terms, masks)])
         assert abs(e - expect) < E_TOL
+    # an operator of Z strings only (one pass over single-element orbits) and
+    # the identity alone
+    zonly = [(0.3 - 0.2j, [("Z", 0)]), (1.5, [("Z", 3), ("Z", 20)]), (-0.7, [("Z", 1), ("Z", 2), ("Z", 9)])]
+    for op in (zonly, [(2.0, [])]):
+        got = qc.direct_op_exp_val(W.to_operator(qf, op))
+        assert abs(got - oracle.expval_masks(psi, [(c,) + tuple(oracle.pauli_masks(t)) for c, t in op])) < E_TOL
     ops = [W.random_pauli_sum(n, 12, seed=80 + k, wmax=4, real_coeffs=False) for k in range(10)]
     want = [oracle.expval_masks(psi, [(c,) + tuple(oracle.pauli_masks(t)) for c, t in op]) for op in ops]
     for _ in range(2):
