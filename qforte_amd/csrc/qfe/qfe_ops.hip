@@ -415,6 +415,17 @@ __device__ __forceinline__ void group_axis(double2 (&a)[8], const uint32_t (&e)[
 // P is evaluated per element from two LDS tables (index bits 0-5 and 6-12),
 // rebuilt once after each run of map updates: two reads and an XOR instead
 // of a GF(2) product over every tile bit.
+// threadIdx.x behind an empty volatile asm: conditions on it inside the op
+// loop of k_fused are evaluated where they are used instead of being hoisted
+// out of the loop as lane masks, each holding two scalar registers for the
+// whole kernel (on 11-qubit tiles those masks pushed the kernel past its
+// register budget into scratch spills).
+__device__ __forceinline__ uint32_t tid_here() {
+    uint32_t t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 template <int KQ>
 struct TileMap {
     // column b of M in lane b of one vector register (every wave keeps the
@@ -477,7 +488,7 @@ struct TileMap {
     // Rebuild the lookup tables after map changes (uniform; every thread calls it).
     __device__ __forceinline__ void refresh(uint32_t* lo, uint32_t* hi) {
         if (!dirty) return;
-        const uint32_t t = threadIdx.x;
+        const uint32_t t = tid_here();
         if (t < 64) lo[t] = lin_cols(t) ^ off;
         if (t < (1u << (KQ > 6 ? KQ - 6 : 0))) hi[t] = lin_cols(t << 6);
         __syncthreads();
@@ -503,7 +514,7 @@ __device__ __forceinline__ void fused_group(double2* tile, const TileMap<KQ>& P,
     const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
     const uint32_t b0 = 1u << t0, b1 = 1u << t1, b2 = 1u << t2;
     const uint32_t p0 = P.lin(b0), p1 = P.lin(b1), p2 = P.lin(b2);
-    for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
+    for (uint32_t it = tid_here(); it < T / 8; it += kFusedTPB) {
         const uint32_t eb = ins0_32(ins0_32(ins0_32(it, lo), mid), hi);
         const uint32_t pb = P(eb);
         uint32_t e[8], pe[8];
@@ -552,7 +563,7 @@ __device__ __forceinline__ void plain_group(double2* tile, const TileMap<KQ>& P,
     const uint32_t lo = min(t0, min(t1, t2)), hi = max(t0, max(t1, t2));
     const uint32_t mid = t0 ^ t1 ^ t2 ^ lo ^ hi;
     const uint32_t p0 = P.lin(1u << t0), p1 = P.lin(1u << t1), p2 = P.lin(1u << t2);
-    for (uint32_t it = threadIdx.x; it < T / 8; it += kFusedTPB) {
+    for (uint32_t it = tid_here(); it < T / 8; it += kFusedTPB) {
         const uint32_t pb = P(ins0_32(ins0_32(ins0_32(it, lo), mid), hi));
         uint32_t pe[8];
         double2 a[8];
@@ -623,7 +634,7 @@ __device__ __forceinline__ void rot_group(double2* tile, const TileMap<KQ>& P, c
     uint32_t pk[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) pk[k] = P.lin(1u << t[k]);
-    for (uint32_t it = threadIdx.x; it < T / N; it += kFusedTPB) {
+    for (uint32_t it = tid_here(); it < T / N; it += kFusedTPB) {
         uint32_t e = it;
 #pragma unroll
         for (int k = 0; k < G; ++k) e = ins0_32(e, srt[k]);
@@ -729,7 +740,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             case K_U1: {
                 const uint32_t tb = 1u << o.t, ptb = P.lin(tb);
                 if (o.hb) {  // X on the target with two or more conditions
-                    for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                    for (uint32_t p = tid_here(); p < T / 2; p += kFusedTPB) {
                         const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
                         const uint32_t q0 = P(l0);
@@ -740,29 +751,34 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
                     break;
                 }
                 const double2 m[4] = {o.m[0], o.m[1], o.m[2], o.m[3]};
-                // the pairs of one op are disjoint: all of a thread's pairs are
-                // read before any is written, so the LDS reads overlap
+                // the pairs of one op are disjoint: a thread reads a batch of
+                // its pairs before writing any, so the LDS reads overlap
+                // (batches of two keep the registers of 11-qubit tiles within
+                // four waves per SIMD without spilling)
                 constexpr uint32_t PP = (T / 2 + kFusedTPB - 1) / kFusedTPB;
-                uint32_t q0[PP];
-                bool on[PP];
-                double2 a[PP], b[PP];
+                constexpr uint32_t PB = PP < 2 ? PP : 2;
+                for (uint32_t r0 = 0; r0 < PP; r0 += PB) {
+                    uint32_t q0[PB];
+                    bool on[PB];
+                    double2 a[PB], b[PB];
 #pragma unroll
-                for (uint32_t r = 0; r < PP; ++r) {
-                    const uint32_t p = threadIdx.x + r * kFusedTPB;
-                    const uint32_t l0 = ins0_32(p, o.t);
-                    q0[r] = P(l0);
-                    on[r] = p < T / 2 && (l0 & (uint32_t)o.cmask) == (uint32_t)o.cval;
-                    if (on[r]) {
-                        a[r] = tile[q0[r]];
-                        b[r] = tile[q0[r] ^ ptb];
+                    for (uint32_t r = 0; r < PB; ++r) {
+                        const uint32_t p = tid_here() + (r0 + r) * kFusedTPB;
+                        const uint32_t l0 = ins0_32(p, o.t);
+                        q0[r] = P(l0);
+                        on[r] = p < T / 2 && (l0 & (uint32_t)o.cmask) == (uint32_t)o.cval;
+                        if (on[r]) {
+                            a[r] = tile[q0[r]];
+                            b[r] = tile[q0[r] ^ ptb];
+                        }
                     }
-                }
 #pragma unroll
-                for (uint32_t r = 0; r < PP; ++r) {
-                    if (on[r]) {
-                        u2x2(a[r], b[r], m);
-                        tile[q0[r]] = a[r];
-                        tile[q0[r] ^ ptb] = b[r];
+                    for (uint32_t r = 0; r < PB; ++r) {
+                        if (on[r]) {
+                            u2x2(a[r], b[r], m);
+                            tile[q0[r]] = a[r];
+                            tile[q0[r] ^ ptb] = b[r];
+                        }
                     }
                 }
                 break;
@@ -770,7 +786,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             case K_DIAG1: {
                 if (o.tin) {
                     const uint32_t ptb = P.lin(1u << o.t);
-                    for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                    for (uint32_t p = tid_here(); p < T / 2; p += kFusedTPB) {
                         const uint32_t l0 = ins0_32(p, o.t);
                         if ((l0 & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
                         const uint32_t q0 = P(l0);
@@ -779,7 +795,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
                     }
                 } else {
                     const double2 f = ((base >> o.t) & 1ull) ? o.m[1] : o.m[0];
-                    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
+                    for (uint32_t l = tid_here(); l < T; l += kFusedTPB) {
                         if ((l & (uint32_t)o.cmask) != (uint32_t)o.cval) continue;
                         const uint32_t q = P(l);
                         tile[q] = cmul(f, tile[q]);
@@ -790,7 +806,7 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
             case K_U2: {
                 const uint32_t lo = o.t < o.c ? o.t : o.c, hi = o.t < o.c ? o.c : o.t;
                 const uint32_t ptb = P.lin(1u << o.t), pcb = P.lin(1u << o.c);
-                for (uint32_t p = threadIdx.x; p < T / 4; p += kFusedTPB) {
+                for (uint32_t p = tid_here(); p < T / 4; p += kFusedTPB) {
                     const uint32_t q = P(ins0_32(ins0_32(p, lo), hi));
                     const uint32_t idx[4] = {q, q ^ ptb, q ^ pcb, q ^ pcb ^ ptb};
                     double2 v[4];
@@ -814,14 +830,14 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
                 if (o.x == 0) {
                     const double2 fp = make_double2(c - sn * ph.y, sn * ph.x);
                     const double2 fm = make_double2(c + sn * ph.y, -sn * ph.x);
-                    for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
+                    for (uint32_t l = tid_here(); l < T; l += kFusedTPB) {
                         const bool neg = ((__popc(l & zl) & 1) != 0) != bneg;
                         const uint32_t q = P(l);
                         tile[q] = cmul(neg ? fm : fp, tile[q]);
                     }
                 } else {
                     const uint32_t xl = (uint32_t)o.x, pxl = P.lin(xl);
-                    for (uint32_t p = threadIdx.x; p < T / 2; p += kFusedTPB) {
+                    for (uint32_t p = tid_here(); p < T / 2; p += kFusedTPB) {
                         const uint32_t la = ins0_32(p, o.hb);
                         const uint32_t lb = la ^ xl;
                         const uint32_t qa = P(la), qb = qa ^ pxl;
