@@ -369,6 +369,7 @@ constexpr uint32_t kGroupPlain = 0x100u;
 // and the head op carries in m[4 + i] the product of the ops' row phases for
 // element i of the group (bit k of i = the value of op k's target).
 constexpr uint32_t kGroupRot = 0x200u;
+constexpr int kWarmOpsMaxTile = 10;  // tiles up to this many qubits touch their ops first
 constexpr int kMaxTileQubits = 13;  // 2^13 * 16 B = 128 KiB of LDS
 
 struct PassDesc {
@@ -703,6 +704,21 @@ __global__ __launch_bounds__(kFusedTPB) __attribute__((amdgpu_waves_per_eu(KQ <=
     for (uint32_t l = threadIdx.x; l < T; l += kFusedTPB) {
         const double2* src = s + (base | hioff[l >> nlow] | (l & lowmask));
         tile[l] = nt ? ld_nt(src) : *src;
+    }
+    if constexpr (KQ <= kWarmOpsMaxTile) {
+        // Small tiles: a workgroup's life is its chain of ops, and each op's
+        // first scalar load would miss the scalar cache (the ops of a pass are
+        // read once per workgroup; 500 gates of any kind took 0.9 us each at
+        // 12 qubits).  Touch the first 144 bytes of every op (its head and
+        // m[0..3]) while the tile loads are in flight, eight ops per wait, so
+        // the chain then hits the cache.
+        uint32_t touch = 0;
+#pragma unroll 8
+        for (uint32_t oi = 0; oi < nops; ++oi) {
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(ops + oi);
+            touch ^= w[0] ^ w[16] ^ w[32];
+        }
+        asm volatile("" ::"s"(touch));
     }
     TileMap<KQ> P;
     P.init(ptab_lo, ptab_hi);
